@@ -1,0 +1,111 @@
+/*
+ * moloch_oracle.h -- CPU FP64 restatement of the MOLOCH timestep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This oracle is the checker for the HIP
+ * implementation in dynamical_model_amd/: only tests/, the smoke() entry
+ * and the cpu_baseline leg of bench.py may load it.  The product never
+ * calls into it.
+ *
+ * It restates the serial (nompi) reference, nompi/mod_moloch.F90, loop by
+ * loop and in the reference's operand order, on plain double arrays.
+ *
+ * Array convention ("canonical layout"): every 2-D/3-D field is a Fortran
+ * array with bounds (-1:nj+2, -1:ni+2 [, 1:nlev]), j fastest.  This is the
+ * union of every halo the reference allocates (ga = 1, gb = 2 points,
+ * mod_atmosphere.F90:104-162); cells a reference array does not own are
+ * never read or written by the restated loops.  Element (j,i,k) sits at
+ * (k-1)*plane + (i+1)*pj + (j+1), pj = nj+4, plane = pj*(ni+4).
+ */
+#ifndef MOLOCH_ORACLE_H
+#define MOLOCH_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ora_state {
+  /* dimensions: jx, iy, kz (nompi/mod_dynparam.F90:30-35) */
+  int nj, ni, nk;
+  /* run parameters (mod_runparams.F90:54-65, nompi/regcm_dynamical_core.F90:99-113) */
+  double dt, dx, dzita, ztop, zh, a0, anu2;
+  int nadv, nsound, nqx, iqfrst;         /* iqfrst is 1-based (iqc = 2) */
+  /* switches */
+  int rotllr;                            /* iproj == 'ROTLLR'               */
+  int divfilter;                         /* mo_divfilter                    */
+  int halo_mpi;                          /* 0: nompi halo order, 1: MPI     */
+  int do_fulleq, do_divdamp, do_vadvtwice, do_filterpai, do_filtertheta;
+
+  /* 1-D: zita(1:kz+1) zitah(1:kz) ffilt(1:kz); derived by ora_init_moloch:
+   * gzitak(1:kz+1) gzitakh(1:kz) xknu(1:kz).  Stored 0-based (k-1). */
+  double *zita, *zitah, *ffilt, *gzitak, *gzitakh, *xknu;
+
+  /* 2-D fields (canonical layout, one plane) */
+  double *hx, *hy, *mu, *mv, *mx, *ulat, *vlat, *ps;
+  double *rmu, *rmv, *mx2, *coru, *corv;        /* derived */
+
+  /* 3-D fields */
+  double *fmz, *fmzf /* kz+1 */, *zeta;
+  double *u, *v, *w /* kz+1 */, *pai, *t, *tetav, *tvirt, *p, *rho, *qsat;
+  double *ux, *vx;
+  double *qx;            /* nqx consecutive kz-level 3-D fields: qv,qc,qi,qr,qs */
+
+  /* module scratch (nompi/mod_moloch.F90:37-91) */
+  double *s /* kz+1 */, *deltaw /* kz+1 */, *zdiv2, *wx, *wz, *p0;
+  double *ud, *vd, *qwltot, *qwitot, *pf, *tf;
+  /* routine-local automatic arrays (:488, :938-940) */
+  double *p2d /* 2-D */, *wwkw /* kz+1 */, *wfw /* kz+1 */, *zpby, *zpbw;
+
+  /* lprint diagnostics (:395-419) */
+  double diag[6];        /* max ps, min ps, max u, min u, max v, min v */
+  int cfl_violation;
+} ora_state;
+
+/* Physical constants of mod_constants.F90, evaluated as the reference does. */
+typedef struct ora_constants {
+  double egrav, rgas, rwat, cpd, cvd, rdrcv, cpovr, rovcp, p00, ep1, ep2;
+  double wlhv, tzero, lrate, eomeg2, degrad, mathpi, regrav, earthrad, raddeg;
+} ora_constants;
+
+void ora_get_constants(ora_constants *c);
+
+/* Saturation (nompi/mod_moloch.F90:119-181) */
+double ora_pfesat(double t, double p);
+double ora_pfwsat(double t, double p);
+
+/* mod_zita.F90 decay function gzita (:110-120), needed by init_moloch. */
+double ora_gzita(double zita, double ztop, double a0);
+
+/* init_moloch / allocate_moloch derived coefficients (:183-268). */
+void ora_init_moloch(ora_state *S);
+
+/* Phases of one moloch call, exposed for unit tests. */
+void ora_reset_tendencies(ora_state *S);            /* :1313-1325 */
+void ora_prologue(ora_state *S);                    /* :291-323 */
+void ora_sound(ora_state *S, double dts);           /* :483-851 */
+void ora_divdamp(ora_state *S, double dts);         /* :853-895 */
+void ora_advection(ora_state *S, double dta);       /* :897-927 */
+void ora_wafone(ora_state *S, double *pp, double dta); /* :929-1311 */
+void ora_uvstagtox(ora_state *S);                   /* :1389-1417 */
+void ora_wstagtox(ora_state *S);                    /* :1327-1341 */
+void ora_xtouvstag(ora_state *S);                   /* :1359-1387 */
+void ora_xtowstag(ora_state *S);                    /* :1343-1357 */
+void ora_epilogue(ora_state *S);                    /* :355-391 */
+void ora_diagnostics(ora_state *S);                 /* :395-419 */
+void ora_filtpai(ora_state *S);                     /* :440-460 */
+void ora_filttheta(ora_state *S);                   /* :462-481 */
+void ora_divergence_filter(ora_state *S);           /* :423-438 */
+
+/* Column tridiagonal elimination exactly as the w solve does it (:634-686),
+ * on one column: given zwexpl(k), zu(k), zd(k) for k=2..kz and the known
+ * boundary values w(1)=w1, w(kz+1)=wkp1, returns w(2..kz).  Arrays are
+ * indexed 0..kz (index k-1 for level k). */
+void ora_thomas_column(int nk, const double *zwexpl, const double *zu,
+                       const double *zd, double w1, double wkp1, double *w);
+
+/* One full moloch(istep, lprint) call (:272-421). */
+void ora_moloch(ora_state *S, int istep, int lprint);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
