@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Throughput of the MOLOCH timestep on one MI355X.
+
+Metric: column-steps/s per GPU -- one (j,i) column advanced by one
+moloch(istep, lprint) call (dt = 30 s: 3 advection cycles x 5 acoustic
+substeps, nompi/mod_moloch.F90:272-421) -- for the plain FP64 core and the
+FP32 core (FP64 Exner function, hydrostatic residuals and column solve), with
+the FP32 core's relative L2 error against the FP64 core after the same steps.
+
+Workload (default): configuration 4, 2048 x 2048 x 60, synthetic initial state
+(standard atmosphere, 20 seeded Gaussian hills, jet, RH 60 %, theta noise)
+built by the library.  Every field is resident in HBM before the timed
+region; the timed region covers exactly --steps calls of the step.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "column-steps/s per GPU: plain FP64 vs FP32+learned acoustic at stated L2 error"
+PROGNOSTIC = ["u", "v", "w", "pai", "tetav", "qv"]
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_variant(dm, hs, case, precision: int, steps: int, warmup: int):
+    params = dm.case_params(case, precision=precision)
+    m = dm.Moloch()
+    m.allocate_moloch(hs.jx, hs.iy, hs.kz, params)
+    m.load_state(hs)
+    m.init_moloch()
+    for n in range(1, warmup + 1):
+        m.moloch(n, False)
+    m.sync()
+    m.profile(True)
+    m.sync()
+    t0 = time.perf_counter()
+    for n in range(warmup + 1, warmup + steps + 1):
+        m.moloch(n, False)
+    m.sync()
+    t1 = time.perf_counter()
+    prof = m.profile_report()
+    m.profile(False)
+    diag = m.moloch(warmup + steps + 1, True)  # one more step with the lprint diagnostics
+    m.sync()
+    finals = {f: dm.interior(m.get_field(f)).copy() for f in PROGNOSTIC}
+    m.close()
+    return (t1 - t0), prof, finals, diag
+
+
+def dominant(prof: dict) -> tuple[str, float, float, int]:
+    name = max(prof, key=lambda k: prof[k][0])
+    ms, n, b = prof[name]
+    return name, ms, b, n
+
+
+def cpu_baseline(jx: int, iy: int, kz: int, steps: int):
+    """The CPU oracle (a single-threaded C restatement of the reference step)
+    on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import dynamical_model_amd as dm
+    import oracle_lib
+
+    case = dm.case_config(4, jx=jx, iy=iy, kz=kz, hill_hwmax=min(40.0, jx / 6.0))
+    hs = dm.build_case(case)
+    o = oracle_lib.oracle_from_host(hs, dm.case_params(case))
+    t0 = time.perf_counter()
+    for n in range(1, steps + 1):
+        o.moloch(n, False)
+    dt = time.perf_counter() - t0
+    return jx * iy * steps / dt, dt
+
+
+def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=4, help="benchmark configuration (grid and physics)")
+    ap.add_argument("--jx", type=int, default=0)
+    ap.add_argument("--iy", type=int, default=0)
+    ap.add_argument("--kz", type=int, default=0)
+    ap.add_argument("--cpu-sample", default="128x128x4", help="JXxIYxSTEPS of the CPU oracle sample, or 'none'")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU halo exchange is a design, not an implementation)", file=sys.stderr)
+        return 2
+
+    import dynamical_model_amd as dm
+
+    over = {k: getattr(args, k) for k in ("jx", "iy", "kz") if getattr(args, k)}
+    case = dm.case_config(args.config, **over)
+    jx, iy, kz = case.jx, case.iy, case.kz
+    t0 = time.perf_counter()
+    hs = dm.build_case(case)
+    log(f"built configuration {args.config} ({jx}x{iy}x{kz}) on the host in {time.perf_counter() - t0:.1f} s")
+
+    results = {}
+    finals = {}
+    for tag, prec in (("fp64", dm.MLC_FP64), ("fp32", dm.MLC_FP32)):
+        t0 = time.perf_counter()
+        secs, prof, fin, diag = run_variant(dm, hs, case, prec, args.steps, args.warmup)
+        cs = jx * iy * args.steps / secs
+        name, ms, b, n = dominant(prof)
+        results[tag] = {
+            "value": cs, "ms_per_step": 1e3 * secs / args.steps, "dominant_kernel": name,
+            "dominant_ms_avg": ms / n, "dominant_gbs": b / (ms * 1e-3) / 1e9,
+            "kernels": {k: {"ms_total": v[0], "launches": v[1], "gbs": (v[2] / (v[0] * 1e-3) / 1e9) if v[0] else 0.0}
+                        for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
+            "diag": {"ps_max": diag.ps_max, "ps_min": diag.ps_min, "u_max": diag.u_max, "u_min": diag.u_min,
+                     "v_max": diag.v_max, "v_min": diag.v_min, "cfl_violation": diag.cfl_violation},
+            "_prof": prof,
+        }
+        finals[tag] = fin
+        log(f"{tag}: {cs:.4g} column-steps/s, {1e3 * secs / args.steps:.2f} ms/step, dominant {name} "
+            f"{results[tag]['dominant_gbs']:.0f} GB/s ({time.perf_counter() - t0:.1f} s incl. set-up)")
+
+    errors = {f: rel_l2(finals["fp32"][f], finals["fp64"][f]) for f in PROGNOSTIC}
+    del finals
+
+    chosen = results["fp32"]
+    name, ms, b, n = dominant(chosen["_prof"])
+    achieved = b / (ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name,
+                "bytes_per_launch": b / n, "avg_launch_ms": ms / n}
+
+    cpu = None
+    if args.cpu_sample != "none":
+        sj, si, ss = (int(x) for x in args.cpu_sample.split("x"))
+        v, dt = cpu_baseline(sj, si, kz, ss)
+        cpu = {"value": v, "unit": "column-steps/s", "cores": 1, "kind": "port",
+               "sample": f"CPU oracle (C restatement of nompi/mod_moloch.F90, FP64, 1 thread), configuration-"
+                         f"{args.config} physics on {sj}x{si}x{kz} for {ss} steps ({dt:.1f} s)"}
+        log(f"cpu baseline {v:.4g} column-steps/s ({dt:.1f} s)")
+
+    for r in results.values():
+        r.pop("_prof")
+    line = {
+        "metric": METRIC,
+        "value": chosen["value"],
+        "unit": "column-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": chosen["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"configuration {args.config}: MOLOCH step on {jx}x{iy}x{kz} "
+                               "(dt=30 s, 3 advection cycles x 5 acoustic substeps), FP32 state with FP64 "
+                               "Exner function, hydrostatic residuals and tridiagonal solve",
+                   "jx": jx, "iy": iy, "kz": kz, "precision": "fp32+fp64-solve"},
+        "fp64": {"value": results["fp64"]["value"], "ms_per_step": results["fp64"]["ms_per_step"], "dtype": "f64"},
+        "speedup_vs_fp64": chosen["value"] / results["fp64"]["value"],
+        "rel_l2_vs_fp64": errors,
+        "rel_l2_after_steps": args.warmup + args.steps + 1,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "variants": results,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -49,7 +49,7 @@ class Params(C.Structure):
         ("rotllr", C.c_int), ("divfilter", C.c_int), ("halo_mpi", C.c_int),
         ("fulleq", C.c_int), ("divdamp", C.c_int), ("vadvtwice", C.c_int),
         ("filterpai", C.c_int), ("filtertheta", C.c_int),
-        ("precision", C.c_int), ("device", C.c_int),
+        ("precision", C.c_int), ("device", C.c_int), ("unfused", C.c_int),
     ]
 
 
@@ -95,6 +95,10 @@ EXPORTS = {
     "mlc_case_params": (C.c_int, [C.POINTER(Case), C.POINTER(Params)]),
     "mlc_case_build": (C.c_int, [C.POINTER(Case), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "mlc_case_load": (C.c_int, [C.c_void_p, C.POINTER(Case)]),
+    "mlc_profile": (C.c_int, [C.c_void_p, C.c_int]),
+    "mlc_profile_kinds": (C.c_int, []),
+    "mlc_profile_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_longlong), C.POINTER(C.c_double)]),
 }
 
 _lib = None

@@ -155,6 +155,24 @@ class Moloch:
     def sync(self) -> None:
         check("mlc_sync", load().mlc_sync(self._h))
 
+    def profile(self, enable: bool) -> None:
+        """Bracket every kernel launch with HIP events (and clear the totals)."""
+        check("mlc_profile", load().mlc_profile(self._h, 1 if enable else 0))
+
+    def profile_report(self) -> dict:
+        """{kernel kind: (total ms, launches, algorithmic bytes)} since profile(True)."""
+        out = {}
+        for k in range(load().mlc_profile_kinds()):
+            name = C.c_char_p()
+            ms = C.c_double()
+            n = C.c_longlong()
+            b = C.c_double()
+            check("mlc_profile_get", load().mlc_profile_get(self._h, k, C.byref(name), C.byref(ms),
+                                                            C.byref(n), C.byref(b)))
+            if n.value:
+                out[name.value.decode()] = (ms.value, n.value, b.value)
+        return out
+
     def close(self) -> None:
         if self._h is not None:
             load().mlc_destroy(self._h)

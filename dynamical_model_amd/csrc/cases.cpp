@@ -59,6 +59,7 @@ struct HostLayout {
 
 // Fill every halo cell of the halo-2 extent periodically from the interior.
 void periodic_fill(const HostLayout &L, double *f, int nlev) {
+#pragma omp parallel for schedule(static)
   for (int k = 1; k <= nlev; k++)
     for (int i = -1; i <= L.iy + 2; i++)
       for (int j = -1; j <= L.jx + 2; j++) {
@@ -222,6 +223,7 @@ extern "C" int mlc_case_build(const mlc_case *c, double *const *fields, double *
       double xc = rng.uniform(0.0, (double)jx), yc = rng.uniform(0.0, (double)iy);
       double hh = rng.uniform(c->hill_hmin, c->hill_hmax);
       double hw = rng.uniform(c->hill_hwmin, c->hill_hwmax);
+#pragma omp parallel for schedule(static)
       for (int i = 1; i <= iy; i++)
         for (int j = 1; j <= jx; j++) {
           double dxp = std::fabs((double)j - xc), dyp = std::fabs((double)i - yc);
@@ -258,6 +260,7 @@ extern "C" int mlc_case_build(const mlc_case *c, double *const *fields, double *
     periodic_fill(L, hy.data(), 1);
   }
   // zeta, fmz (:167-172), fmzf (:185-190)
+#pragma omp parallel for schedule(static)
   for (int k = 1; k <= kz; k++)
     for (int i = 1; i <= iy; i++)
       for (int j = 1; j <= jx; j++) {
@@ -266,6 +269,7 @@ extern "C" int mlc_case_build(const mlc_case *c, double *const *fields, double *
         fmz[L.at(j, i, k)] = md_fmz(zitah[k - 1], g, c->ztop, c->zh, c->a0);
         zabs[L.at(j, i, k)] = zeta[L.at(j, i, k)] + g * kRegrav;
       }
+#pragma omp parallel for schedule(static)
   for (int k = 1; k <= kz + 1; k++)
     for (int i = 1; i <= iy; i++)
       for (int j = 1; j <= jx; j++) fmzf[L.at(j, i, k)] = md_fmz(zita[k - 1], ht[L.at(j, i)], c->ztop, c->zh, c->a0);
@@ -316,6 +320,7 @@ extern "C" int mlc_case_build(const mlc_case *c, double *const *fields, double *
     }
   } else {
     StdAtm A{c->tsfc, c->lapse, c->ttrop, c->psfc};
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= kz; k++)
       for (int i = 1; i <= iy; i++)
         for (int j = 1; j <= jx; j++) {
@@ -327,6 +332,7 @@ extern "C" int mlc_case_build(const mlc_case *c, double *const *fields, double *
     for (int i = 1; i <= iy; i++)
       for (int j = 1; j <= jx; j++) ps[L.at(j, i)] = A.p(ht[L.at(j, i)] * kRegrav);
     // wind: u on the west face of cell j, heights averaged over j-1 and j
+#pragma omp parallel for schedule(static)
     for (int k = 1; k <= kz; k++)
       for (int i = 1; i <= iy; i++)
         for (int j = 1; j <= jx; j++) {
@@ -343,6 +349,7 @@ extern "C" int mlc_case_build(const mlc_case *c, double *const *fields, double *
   }
 
   // discrete hydrostatic Exner function (mod_atmosphere.F90:210-234)
+#pragma omp parallel for schedule(static)
   for (int i = 1; i <= iy; i++)
     for (int j = 1; j <= jx; j++) {
       long xk = L.at(j, i, kz), xk1 = L.at(j, i, kz - 1);
@@ -357,6 +364,7 @@ extern "C" int mlc_case_build(const mlc_case *c, double *const *fields, double *
       pai[xk] = std::pow(p / kP00, kRovcp);
     }
   for (int k = kz - 1; k >= 1; k--)
+#pragma omp parallel for schedule(static)
     for (int i = 1; i <= iy; i++)
       for (int j = 1; j <= jx; j++) {
         long x = L.at(j, i, k), xp = L.at(j, i, k + 1);
@@ -381,6 +389,7 @@ extern "C" int mlc_case_build(const mlc_case *c, double *const *fields, double *
     }
     if (c->bubble_amp != 0.0) {
       double jc = 0.5 * (double)(jx + 1), ic = 0.5 * (double)(iy + 1);
+#pragma omp parallel for schedule(static)
       for (int k = 1; k <= kz; k++)
         for (int i = 1; i <= iy; i++)
           for (int j = 1; j <= jx; j++) {

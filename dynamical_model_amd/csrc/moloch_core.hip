@@ -91,6 +91,7 @@ struct Dev {
   const double *gzitak, *gzitakh, *xknu, *ffilt;
   // scratch
   T *s, *deltaw, *zdh, *zds, *zdf, *wx, *qwltot, *qwitot, *tf, *sm;
+  T *spare;               // output buffer of the fused advection (swapped in)
   double *wz, *p0;        // advection scratch, sized for an FP64 field
   double *pf, *smd;       // filter scratch
   double *thg, *thc;      // tridiagonal elimination (FP64)
@@ -717,6 +718,255 @@ __global__ void __launch_bounds__(256) k_waf_z(Dev<T> D, Par P, Q *pp, const Q *
   pp[x] = out;
 }
 
+// ========================================================== fused wafone
+// One launch advects one field through the two vertical half passes, the
+// meridional and the zonal sweep (:929-1311 with do_vadvtwice).  A block
+// owns a tile of WJ x WI columns and marches down the levels.  Its threads
+// run the vertical passes as a register pipeline on the tile extended by two
+// columns on every side (the reach of the horizontal sweeps), so output
+// level k is ready when input level k+4 arrives; each ready level goes
+// through LDS planes -- vertical result, original field and fmz -- from
+// which the meridional sweep builds p0 on the rows of the extended tile and
+// the zonal sweep the result on the tile.  Halo columns are the periodic
+// images, i.e. the values the reference's halo copies hold (:1075-1078,
+// :1133-1136).  Arithmetic and operand order are those of the unfused
+// kernels.  The result goes to a second buffer (pp_out).
+constexpr int WJ = 28, WI = 12;            // interior tile
+constexpr int EJ = WJ + 4, EI = WI + 4;    // extended tile (32 x 16)
+constexpr int WTHREADS = 256;              // 32 x 8 threads, 2 columns each
+
+// Per-column state of the vertical pipeline.  Windows are indexed by level
+// relative to the input level m of the current iteration: slot 4 = level m,
+// slot 0 = level m-4 (P, S, ZU, ZD, FZ); W holds first-pass values of
+// levels m-5..m-2 (slot 3 = m-2).  Levels outside 1..nk hold the clamped
+// values the reference's k1/k1p1 clamps select (:980-990).
+template <typename Q>
+struct VPipe {
+  Q P[5], S[5], ZU[5], ZD[5], FZ[5];
+  Q W[4];
+  Q F1, F2;   // flux at the upper interface of the level the pass finishes next
+  Q ffz;      // fmzf of level m
+};
+
+// WAF flux through interface m (between levels m-1 above and m below),
+// :966-1001 written for a window: qm2..q1 are levels m-2..m+1, clamped.
+template <typename Q>
+__device__ __forceinline__ Q waf_vflux(Q s, Q dtrdz, Q qm2, Q qm1, Q q0, Q q1) {
+  Q zamu = s * dtrdz, is, a, b;
+  if (zamu >= Q(0)) {
+    is = Q(1);
+    a = q0;
+    b = q1;
+  } else {
+    is = Q(-1);
+    a = qm2;
+    b = qm1;
+  }
+  Q zzden = sgn_floor(qm1 - q0);
+  Q r = (a - b) / zzden;
+  Q bb = waf_b(r);
+  Q zphi = is + zamu * bb - is * bb;
+  return Q(0.5) * s * ((Q(1) + zphi) * q0 + (Q(1) - zphi) * qm1);
+}
+
+template <typename T, typename Q>
+__global__ void __launch_bounds__(WTHREADS) k_waf_fused(Dev<T> D, Par P, const Q *__restrict__ pp,
+                                                         Q *__restrict__ out, double dta) {
+  __shared__ Q sW[2][EI][EJ];
+  __shared__ Q sP[2][EI][EJ];
+  __shared__ Q sF[2][EI][EJ];
+  __shared__ Q sP0[2][WI][EJ];
+  const Geo &g = D.g;
+  const int nj = g.nj, ni = g.ni, nk = g.nk;
+  const int tx = threadIdx.x % EJ, ty = threadIdx.x / EJ;  // ty in 0..7
+  const int j0 = blockIdx.x * WJ + 1, i0 = blockIdx.y * WI + 1;
+  const Q dtrdz = Q(0.5) * (Q)(dta / P.dzita);
+  const Q dtrdx = (Q)(dta / P.dx), dtrdy = (Q)(dta / P.dx);
+  const int imin = -1, imax = ni + 2, jmin = -1, jmax = nj + 2;  // :264-267
+
+  long cb[2];
+  VPipe<Q> V[2];
+  for (int c = 0; c < 2; c++) {
+    int jw = ((j0 - 2 + tx - 1) % nj + nj) % nj + 1;
+    int iw = ((i0 - 2 + ty + 8 * c - 1) % ni + ni) % ni + 1;
+    cb[c] = g.at(jw, iw, 1);
+  }
+
+  for (int m = 1; m <= nk + 4; m++) {
+    const int kb = m - 2, kc = m - 4;
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      VPipe<Q> &v = V[c];
+      const long c0 = cb[c];
+      // ---- receive level m (clamped to nk) and interface m
+      const int mm = m <= nk ? m : nk;
+      const long xm = c0 + (long)(mm - 1) * g.plane;
+      Q pm = pp[xm];
+      Q fzm = (Q)D.fmz[xm];
+      Q sm = m <= nk + 1 ? (Q)D.s[c0 + (long)(m - 1) * g.plane] : Q(0);
+      if (m == 1) {
+        v.ffz = (Q)D.fmzf[c0];
+#pragma unroll
+        for (int q = 0; q < 5; q++) v.P[q] = pm, v.S[q] = Q(0), v.FZ[q] = fzm, v.ZU[q] = Q(0), v.ZD[q] = Q(0);
+        v.F1 = Q(0);
+        v.F2 = Q(0);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        v.P[q] = v.P[q + 1];
+        v.S[q] = v.S[q + 1];
+        v.FZ[q] = v.FZ[q + 1];
+        v.ZU[q] = v.ZU[q + 1];
+        v.ZD[q] = v.ZD[q + 1];
+      }
+      v.P[4] = pm;
+      v.S[4] = sm;
+      v.FZ[4] = fzm;
+      if (m <= nk) {
+        Q ff1 = (Q)D.fmzf[xm + g.plane];
+        v.ZU[4] = dtrdz * fzm / v.ffz;  // zrfmu (:1010)
+        v.ZD[4] = dtrdz * fzm / ff1;    // zrfmd (:1011)
+        v.ffz = ff1;
+      }
+      // ---- first pass: level kb = m-2 from interfaces m-2 (F1) and m-1
+      if (kb >= 1 && kb <= nk) {
+        Q f1 = Q(0);
+        if (m - 1 >= 2 && m - 1 <= nk) f1 = waf_vflux(v.S[3], dtrdz, v.P[1], v.P[2], v.P[3], v.P[4]);
+        Q q = v.P[2];
+        Q zdv = (v.S[2] * v.ZU[2] - v.S[3] * v.ZD[2]) * q;
+        Q w1 = q - v.F1 * v.ZU[2] + f1 * v.ZD[2] + zdv;
+        v.F1 = f1;
+        if (kb == 1) {
+          v.W[0] = v.W[1] = v.W[2] = v.W[3] = w1;
+        } else {
+          v.W[0] = v.W[1];
+          v.W[1] = v.W[2];
+          v.W[2] = v.W[3];
+          v.W[3] = w1;
+        }
+      } else if (kb > nk) {
+        v.W[0] = v.W[1];
+        v.W[1] = v.W[2];
+        v.W[2] = v.W[3];  // level kb clamps to nk
+      }
+      // ---- second pass: level kc = m-4 from interfaces m-4 (F2) and m-3
+      if (kc >= 1 && kc <= nk) {
+        Q f2 = Q(0);
+        if (m - 3 >= 2 && m - 3 <= nk) f2 = waf_vflux(v.S[1], dtrdz, v.W[0], v.W[1], v.W[2], v.W[3]);
+        Q q = v.W[1];
+        Q zdv = (v.S[0] * v.ZU[0] - v.S[1] * v.ZD[0]) * q;
+        Q w2 = q - v.F2 * v.ZU[0] + f2 * v.ZD[0] + zdv;
+        v.F2 = f2;
+        const int ei = ty + 8 * c, b = kc & 1;
+        sW[b][ei][tx] = w2;
+        sP[b][ei][tx] = v.P[0];
+        sF[b][ei][tx] = v.FZ[0];
+      }
+    }
+    if (kc < 1) continue;  // uniform across the block
+    const int b = kc & 1;
+    __syncthreads();
+    // ---- meridional sweep on the rows of the extended tile -> p0
+    for (int q = threadIdx.x; q < EJ * WI; q += WTHREADS) {
+      const int ej = q % EJ, it = q / EJ;
+      const int i = i0 + it;
+      if (i > ni) continue;
+      const int jw = ((j0 - 2 + ej - 1) % nj + nj) % nj + 1;
+      const int ec = it + 2;  // extended row of cell i
+      auto fl = [&](int ii, int er) -> Q {  // flux at south face of cell ii (extended row er)
+        int iw2 = ii > ni ? ii - ni : ii;
+        Q vv = (Q)D.v[g.at(jw, iw2, kc)];
+        Q zamu = P.rotllr ? vv * dtrdy : vv * (Q)D.rmv[g.at2(jw, iw2)] * dtrdy;
+        Q is;
+        int ih;
+        if (zamu > Q(0)) {
+          is = Q(1);
+          ih = ii - 1;
+        } else {
+          is = Q(-1);
+          ih = ii + 1 < imax ? ii + 1 : imax;
+        }
+        int ihm1 = ih - 1 > imin ? ih - 1 : imin;
+        Q wc = sW[b][er][ej], wm = sW[b][er - 1][ej];
+        Q zzden = sgn_floor(wc - wm);
+        Q r = (sW[b][er + (ih - ii)][ej] - sW[b][er + (ihm1 - ii)][ej]) / zzden;
+        Q bb = waf_b(r);
+        Q zphi = is + zamu * bb - is * bb;
+        if (P.rotllr) return Q(0.5) * vv * ((Q(1) + zphi) * wm + (Q(1) - zphi) * wc);
+        return Q(0.5) * vv * (Q)D.rmv[g.at2(jw, iw2)] * ((Q(1) + zphi) * wm + (Q(1) - zphi) * wc);
+      };
+      Q zs = fl(i, ec), zn = fl(i + 1, ec + 1);
+      Q f0 = sF[b][ec][ej];
+      const int ip = i + 1 > ni ? 1 : i + 1;
+      Q vc = (Q)D.v[g.at(jw, i, kc)], vp = (Q)D.v[g.at(jw, ip, kc)];
+      Q o;
+      if (P.rotllr) {
+        Q mx = (Q)D.mx[g.at2(jw, i)];
+        Q zhxvtn = dtrdy * (Q)D.rmv[g.at2(jw, ip)] * mx;
+        Q zhxvts = dtrdy * (Q)D.rmv[g.at2(jw, i)] * mx;
+        Q zrfmn = zhxvtn * Q(2) * f0 / (f0 + sF[b][ec + 1][ej]);
+        Q zrfms = zhxvts * Q(2) * f0 / (f0 + sF[b][ec - 1][ej]);
+        Q zdv = (vp * zrfmn - vc * zrfms) * sP[b][ec][ej];
+        o = sW[b][ec][ej] + zs * zrfms - zn * zrfmn + zdv;
+      } else {
+        Q zrfmn = dtrdy * Q(2) * f0 / (f0 + sF[b][ec + 1][ej]);
+        Q zrfms = dtrdy * Q(2) * f0 / (f0 + sF[b][ec - 1][ej]);
+        Q zdv = (vp * (Q)D.rmv[g.at2(jw, ip)] * zrfmn - vc * (Q)D.rmv[g.at2(jw, i)] * zrfms) * sP[b][ec][ej];
+        o = sW[b][ec][ej] + (Q)D.mx2[g.at2(jw, i)] * (zs * zrfms - zn * zrfmn + zdv);
+      }
+      sP0[b][it][ej] = o;
+    }
+    __syncthreads();
+    // ---- zonal sweep on the tile -> out
+    for (int q = threadIdx.x; q < WJ * WI; q += WTHREADS) {
+      const int jt = q % WJ, it = q / WJ;
+      const int j = j0 + jt, i = i0 + it;
+      if (j > nj || i > ni) continue;
+      const int ec = jt + 2, er = it + 2;
+      auto fl = [&](int jj, int e) -> Q {  // flux at west face of cell jj (extended column e)
+        int jw2 = jj > nj ? jj - nj : jj;
+        Q uu = (Q)D.u[g.at(jw2, i, kc)];
+        Q zamu = P.rotllr ? uu * (Q)D.mu[g.at2(jw2, i)] * dtrdx : uu * (Q)D.rmu[g.at2(jw2, i)] * dtrdx;
+        Q is;
+        int jh;
+        if (zamu > Q(0)) {
+          is = Q(1);
+          jh = jj - 1;
+        } else {
+          is = Q(-1);
+          jh = jj + 1 < jmax ? jj + 1 : jmax;
+        }
+        int jhm1 = jh - 1 > jmin ? jh - 1 : jmin;
+        Q pc = sP0[b][it][e], pm = sP0[b][it][e - 1];
+        Q zzden = sgn_floor(pc - pm);
+        Q r = (sP0[b][it][e + (jh - jj)] - sP0[b][it][e + (jhm1 - jj)]) / zzden;
+        Q bb = waf_b(r);
+        Q zphi = is + zamu * bb - is * bb;
+        if (P.rotllr) return Q(0.5) * uu * ((Q(1) + zphi) * pm + (Q(1) - zphi) * pc);
+        return Q(0.5) * uu * (Q)D.rmu[g.at2(jw2, i)] * ((Q(1) + zphi) * pm + (Q(1) - zphi) * pc);
+      };
+      Q zw = fl(j, ec), ze = fl(j + 1, ec + 1);
+      Q f0 = sF[b][er][ec];
+      const int jp = j + 1 > nj ? 1 : j + 1;
+      Q uc = (Q)D.u[g.at(j, i, kc)], up = (Q)D.u[g.at(jp, i, kc)];
+      Q o;
+      if (P.rotllr) {
+        Q zcostx = dtrdx * (Q)D.mu[g.at2(j, i)];
+        Q zrfme = zcostx * Q(2) * f0 / (f0 + sF[b][er][ec + 1]);
+        Q zrfmw = zcostx * Q(2) * f0 / (f0 + sF[b][er][ec - 1]);
+        Q zdv = (up * zrfme - uc * zrfmw) * sP[b][er][ec];
+        o = sP0[b][it][ec] + zw * zrfmw - ze * zrfme + zdv;
+      } else {
+        Q zrfme = dtrdx * Q(2) * f0 / (f0 + sF[b][er][ec + 1]);
+        Q zrfmw = dtrdx * Q(2) * f0 / (f0 + sF[b][er][ec - 1]);
+        Q zdv = (up * (Q)D.rmu[g.at2(jp, i)] * zrfme - uc * (Q)D.rmu[g.at2(j, i)] * zrfmw) * sP[b][er][ec];
+        o = sP0[b][it][ec] + (Q)D.mx2[g.at2(j, i)] * (zw * zrfmw - ze * zrfme + zdv);
+      }
+      out[g.at(j, i, kc)] = o;
+    }
+  }
+}
+
 // ================================================================ epilogue
 // :355-391 -- tvirt, t, p, rho, ps (level-kz extrapolation), qsat.
 template <typename T>
@@ -847,6 +1097,15 @@ __global__ void __launch_bounds__(256) k_diag_final(double *partial, long nblk, 
 }
 
 // ============================================================ host driver
+// Kernel kinds for the per-launch event timing of mlc_profile_*.
+enum KernelKind {
+  KK_HALO = 0, KK_PROLOGUE, KK_SOUND_A, KK_SOUND_B, KK_SOUND_B2, KK_SOUND_C, KK_SMOOTH5, KK_FILTER_BRACKET,
+  KK_ADV_IN, KK_WAF_V, KK_WAF_M, KK_WAF_Z, KK_WAF_FUSED, KK_ADV_OUT, KK_EPILOGUE, KK_DIAG, KK_COUNT
+};
+static const char *kKernelNames[KK_COUNT] = {
+    "halo", "prologue", "sound_a", "sound_b", "sound_b2", "sound_c", "smooth5", "filter_bracket",
+    "adv_in", "waf_v", "waf_m", "waf_z", "waf_fused", "adv_out", "epilogue", "diag"};
+
 struct CoreBase {
   virtual ~CoreBase() {}
   virtual int set_field(int f, const double *host, const int *lb, const int *ub) = 0;
@@ -854,6 +1113,8 @@ struct CoreBase {
   virtual int init() = 0;
   virtual int step(int istep, int lprint, mlc_diag *diag) = 0;
   virtual int sync() = 0;
+  virtual int profile(int enable) = 0;
+  virtual int profile_get(int kind, double *total_ms, long long *launches, double *bytes) = 0;
 };
 
 static long round_up(long a, long b) { return (a + b - 1) / b * b; }
@@ -869,8 +1130,89 @@ struct Core : CoreBase {
   std::vector<double> zita, zitah, ffilt;                   // host levels
   std::vector<double> h_msfu, h_msfv, h_msfx, h_ulat, h_vlat;  // host geometry
   bool initialised = false;
+  bool unfused_advection = false;  // params.unfused: the three-sweep wafone kernels
   long nblk_diag = 0;
   double *d_diag_out = nullptr;
+
+  // ---------------------------------------------- per-kernel event timing
+  struct ProfRec {
+    int kind;
+    double bytes;
+    hipEvent_t a, b;
+  };
+  bool prof = false;
+  std::vector<hipEvent_t> evpool;
+  size_t evnext = 0;
+  std::vector<ProfRec> pending;
+  double acc_ms[KK_COUNT] = {0};
+  long long acc_n[KK_COUNT] = {0};
+  double acc_bytes[KK_COUNT] = {0};
+
+  // Algorithmic bytes of one launch: nT passes over kz-level fields of the
+  // storage type, nD passes over FP64 fields (the Exner function).
+  double fbytes(double nT, double nD) const {
+    return (double)g.nj * g.ni * g.nk * (nT * sizeof(T) + nD * sizeof(double));
+  }
+
+  int resolve() {
+    if (pending.empty()) return MLC_OK;
+    HIP_TRY(hipEventSynchronize(pending.back().b));
+    for (auto &r : pending) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+      acc_ms[r.kind] += ms;
+      acc_n[r.kind] += 1;
+      acc_bytes[r.kind] += r.bytes;
+    }
+    pending.clear();
+    evnext = 0;
+    return MLC_OK;
+  }
+
+  template <typename Kern, typename... Args>
+  void launch(int kind, double bytes, Kern kern, dim3 grid, dim3 block, Args... args) {
+    ProfRec r{kind, bytes, nullptr, nullptr};
+    if (prof) {
+      if (evnext + 2 > evpool.size()) {
+        if (evpool.size() < 16384) {
+          for (int n = 0; n < 256; n++) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) break;
+            evpool.push_back(e);
+          }
+        }
+        if (evnext + 2 > evpool.size()) (void)resolve();
+      }
+      if (evnext + 2 <= evpool.size()) {
+        r.a = evpool[evnext++];
+        r.b = evpool[evnext++];
+        (void)hipEventRecord(r.a, stream);
+      }
+    }
+    hipLaunchKernelGGL(kern, grid, block, 0, stream, args...);
+    if (prof && r.a) {
+      (void)hipEventRecord(r.b, stream);
+      pending.push_back(r);
+    }
+  }
+
+  int profile(int enable) override {
+    int rc = resolve();
+    if (rc) return rc;
+    prof = enable != 0;
+    for (int k = 0; k < KK_COUNT; k++) acc_ms[k] = 0, acc_n[k] = 0, acc_bytes[k] = 0;
+    return MLC_OK;
+  }
+
+  int profile_get(int kind, double *total_ms, long long *launches, double *bytes) override {
+    if (kind < 0 || kind >= KK_COUNT) return mlc_fail(MLC_EINVAL, "mlc_profile_get: kind out of range");
+    int rc = resolve();
+    if (rc) return rc;
+    if (total_ms) *total_ms = acc_ms[kind];
+    if (launches) *launches = acc_n[kind];
+    if (bytes) *bytes = acc_bytes[kind];
+    return MLC_OK;
+  }
 
   int alloc_field(T **p, int nlev) { return alloc_bytes((void **)p, (size_t)g.plane * nlev * sizeof(T)); }
   int alloc_d(double **p, long n) { return alloc_bytes((void **)p, (size_t)n * sizeof(double)); }
@@ -885,6 +1227,7 @@ struct Core : CoreBase {
 
   ~Core() override {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
     for (void *p : allocs) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -892,6 +1235,7 @@ struct Core : CoreBase {
   int create(const mlc_dims &d, const mlc_params &p) {
     dims = d;
     prm = p;
+    unfused_advection = p.unfused != 0;
     HIP_TRY(hipSetDevice(p.device));
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     g.nj = d.jx;
@@ -909,7 +1253,7 @@ struct Core : CoreBase {
     AF(fmz, K) AF(fmzf, K1) AF(zeta, K) AF(hx, 1) AF(hy, 1) AF(mu, 1) AF(mv, 1) AF(mx, 1);
     AF(rmu, 1) AF(rmv, 1) AF(mx2, 1) AF(coru, 1) AF(corv, 1);
     AF(s, K1) AF(deltaw, K1) AF(zdh, K) AF(zds, K) AF(zdf, K) AF(wx, K) AF(qwltot, K) AF(qwitot, K);
-    AF(tf, K) AF(sm, K);
+    AF(tf, K) AF(sm, K) AF(spare, K);
 #undef AF
     if ((rc = alloc_d(&D.pai, g.plane * K)) != MLC_OK) return rc;
     if ((rc = alloc_d(&D.wz, g.plane * K)) != MLC_OK) return rc;
@@ -999,6 +1343,7 @@ struct Core : CoreBase {
     int j0 = l[0] > -1 ? l[0] : -1, j1 = u[0] < g.nj + 2 ? u[0] : g.nj + 2;
     int i0 = l[1] > -1 ? l[1] : -1, i1 = u[1] < g.ni + 2 ? u[1] : g.ni + 2;
     int k0 = l[2] > 1 ? l[2] : 1, k1 = u[2] < s.nlev ? u[2] : s.nlev;
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = k0; k <= k1; k++)
       for (int i = i0; i <= i1; i++)
         for (int j = j0; j <= j1; j++) {
@@ -1023,18 +1368,26 @@ struct Core : CoreBase {
       if (!s.dev) return MLC_OK;
     }
     size_t n = (size_t)g.plane * s.nlev;
+    // a host array covering the whole device extent needs no read-back
+    const int rank = s.nlev == 1 ? 2 : 3;
+    const bool covers = lb[0] <= -1 && lb[1] <= -1 && ub[0] >= g.nj + 2 && ub[1] >= g.ni + 2 &&
+                        (rank == 2 || (lb[2] <= 1 && ub[2] >= s.nlev));
     if (s.is_double) {
       std::vector<double> buf(n);
-      HIP_TRY(hipMemcpyAsync(buf.data(), s.dev, n * sizeof(double), hipMemcpyDeviceToHost, stream));
-      HIP_TRY(hipStreamSynchronize(stream));
+      if (!covers) {
+        HIP_TRY(hipMemcpyAsync(buf.data(), s.dev, n * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+      }
       rc = visit(s, lb, ub, [&](long h, long d) { buf[d] = host[h]; });
       if (rc) return rc;
       HIP_TRY(hipMemcpyAsync(s.dev, buf.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
       HIP_TRY(hipStreamSynchronize(stream));
     } else {
       std::vector<T> buf(n);
-      HIP_TRY(hipMemcpyAsync(buf.data(), s.dev, n * sizeof(T), hipMemcpyDeviceToHost, stream));
-      HIP_TRY(hipStreamSynchronize(stream));
+      if (!covers) {
+        HIP_TRY(hipMemcpyAsync(buf.data(), s.dev, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+      }
       rc = visit(s, lb, ub, [&](long h, long d) { buf[d] = (T)host[h]; });
       if (rc) return rc;
       HIP_TRY(hipMemcpyAsync(s.dev, buf.data(), n * sizeof(T), hipMemcpyHostToDevice, stream));
@@ -1127,7 +1480,7 @@ struct Core : CoreBase {
   template <typename F>
   void halo(F *f, int nlev, int mode) {
     int n = (g.nj > g.ni ? g.nj : g.ni) + 1;
-    hipLaunchKernelGGL(k_halo<F>, dim3((n + 255) / 256, nlev), dim3(256), 0, stream, f, g, mode);
+    launch(KK_HALO, 0.0, k_halo<F>, dim3((n + 255) / 256, nlev), dim3(256), f, g, mode);
   }
 
   void sound(double dts) {
@@ -1136,51 +1489,67 @@ struct Core : CoreBase {
     for (int ns = 1; ns <= prm.nsound; ns++) {
       halo(D.u, g.nk, 1);
       halo(D.v, g.nk, 1);
-      hipLaunchKernelGGL(k_sound_a<T>, col_grid(), blk(), 0, stream, D, P, dts);
+      launch(KK_SOUND_A, fbytes(5, 0), k_sound_a<T>, col_grid(), blk(), D, P, dts);
       if (!prm.divdamp) {
         // without divdamp the undamped divergence feeds the column directly;
         // express it as divdamp with zero coefficients is not equivalent, so
         // refuse at create time instead (see mlc_create).
       }
       int split = prm.divfilter ? 1 : 0;
-      hipLaunchKernelGGL(k_sound_b<T>, col_grid(), blk(), 0, stream, D, P, dts, ns, split);
+      launch(KK_SOUND_B, fbytes(18, 2), k_sound_b<T>, col_grid(), blk(), D, P, dts, ns, split);
       if (split) {
         // divergence_filter (:423-438) then the Exner update
         halo(D.zds, g.nk, 0);
-        hipLaunchKernelGGL(k_smooth5<T>, pt_grid(), blk(), 0, stream, (const T *)D.zds, D.zdf, g, D.xknu, prm.anu2);
-        hipLaunchKernelGGL(k_sound_b2<T>, col_grid(), blk(), 0, stream, D, P);
+        launch(KK_SMOOTH5, fbytes(2, 0), k_smooth5<T>, pt_grid(), blk(), (const T *)D.zds, D.zdf, g,
+               (const double *)D.xknu, prm.anu2);
+        launch(KK_SOUND_B2, fbytes(1, 2), k_sound_b2<T>, col_grid(), blk(), D, P);
       }
-      hipLaunchKernelGGL(k_sound_c<T>, col_grid(), blk(), 0, stream, D, P, dts);
+      launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T>, col_grid(), blk(), D, P, dts);
       std::swap(D.u, D.un);
       std::swap(D.v, D.vn);
     }
   }
 
+  // wafone (:929-1311) on the field pointed to by pp.  With the two vertical
+  // half passes the fused kernel writes the result to the spare buffer of
+  // the field's type, which then takes the field's place.
   template <typename Q>
-  void wafone(Q *pp, double dta) {
+  void wafone(Q *&pp, Q *&spare, double dta) {
     Par P = par();
+    const double q = sizeof(Q) == sizeof(double) ? 1.0 : 0.0;  // advected field in FP64?
+    if (prm.vadvtwice && !unfused_advection) {
+      // pp, s, fmz, fmzf, u, v -> result
+      dim3 grid((g.nj + WJ - 1) / WJ, (g.ni + WI - 1) / WI, 1);
+      launch(KK_WAF_FUSED, fbytes(4 + 2 * (1 - q), 2 * q), k_waf_fused<T, Q>, grid, dim3(WTHREADS), D, P,
+             (const Q *)pp, spare, dta);
+      std::swap(pp, spare);
+      return;
+    }
     Q *wz = (Q *)D.wz, *p0 = (Q *)D.p0;
-    hipLaunchKernelGGL((k_waf_v<T, Q>), col_grid(), blk(), 0, stream, D, P, (const Q *)pp, wz, dta,
-                       prm.vadvtwice ? 1 : 0);
-    hipLaunchKernelGGL((k_waf_m<T, Q>), pt_grid(), blk(), 0, stream, D, P, (const Q *)pp, (const Q *)wz, p0, dta);
-    hipLaunchKernelGGL((k_waf_z<T, Q>), pt_grid(), blk(), 0, stream, D, P, pp, (const Q *)p0, dta);
+    // vertical: pp, s, fmz, fmzf -> wz;  meridional: wz, v, fmz, pp -> p0;
+    // zonal: p0, u, fmz, pp -> pp
+    launch(KK_WAF_V, fbytes(3 + 2 * (1 - q), 2 * q), k_waf_v<T, Q>, col_grid(), blk(), D, P, (const Q *)pp, wz, dta,
+           prm.vadvtwice ? 1 : 0);
+    launch(KK_WAF_M, fbytes(2 + 3 * (1 - q), 3 * q), k_waf_m<T, Q>, pt_grid(), blk(), D, P, (const Q *)pp,
+           (const Q *)wz, p0, dta);
+    launch(KK_WAF_Z, fbytes(2 + 3 * (1 - q), 3 * q), k_waf_z<T, Q>, pt_grid(), blk(), D, P, pp, (const Q *)p0, dta);
   }
 
   void advection(double dta) {
     halo(D.u, g.nk, 2);
     halo(D.v, g.nk, 3);
-    hipLaunchKernelGGL(k_adv_in<T>, col_grid(), blk(), 0, stream, D);
-    wafone<T>(D.tetav, dta);
-    wafone<double>(D.pai, dta);
-    wafone<T>(D.ux, dta);
-    wafone<T>(D.vx, dta);
-    wafone<T>(D.wx, dta);
-    wafone<T>(D.qv, dta);
-    T *q[5] = {D.qv, D.qc, D.qi, D.qr, D.qs};
-    for (int n = prm.iqfrst; n <= dims.nqx; n++) wafone<T>(q[n - 1], dta);
+    launch(KK_ADV_IN, fbytes(9, 0), k_adv_in<T>, col_grid(), blk(), D);
+    wafone<T>(D.tetav, D.spare, dta);
+    wafone<double>(D.pai, D.smd, dta);
+    wafone<T>(D.ux, D.spare, dta);
+    wafone<T>(D.vx, D.spare, dta);
+    wafone<T>(D.wx, D.spare, dta);
+    wafone<T>(D.qv, D.spare, dta);
+    T *Dev<T>::*qm[5] = {&Dev<T>::qv, &Dev<T>::qc, &Dev<T>::qi, &Dev<T>::qr, &Dev<T>::qs};
+    for (int n = prm.iqfrst; n <= dims.nqx; n++) wafone<T>(D.*qm[n - 1], D.spare, dta);
     halo(D.ux, g.nk, 2);
     halo(D.vx, g.nk, 3);
-    hipLaunchKernelGGL(k_adv_out<T>, col_grid(), blk(), 0, stream, D);
+    launch(KK_ADV_OUT, fbytes(6, 0), k_adv_out<T>, col_grid(), blk(), D);
   }
 
   int step(int istep, int lprint, mlc_diag *diag) override {
@@ -1190,32 +1559,33 @@ struct Core : CoreBase {
     const double dtstepa = prm.dt / (double)prm.nadv;
     const double dtsound = dtstepa / (double)prm.nsound;
     Par P = par();
-    hipLaunchKernelGGL(k_prologue<T>, col_grid(), blk(), 0, stream, D, P, prm.filterpai ? 1 : 0,
-                       (prm.fulleq && prm.filtertheta) ? 1 : 0);
+    launch(KK_PROLOGUE, fbytes(10, 1), k_prologue<T>, col_grid(), blk(), D, P, prm.filterpai ? 1 : 0,
+           (prm.fulleq && prm.filtertheta) ? 1 : 0);
     for (int n = 1; n <= prm.nadv; n++) {
       sound(dtsound);
       advection(dtstepa);
     }
     if (prm.filterpai) {  // :333-341
-      hipLaunchKernelGGL((k_sub<double, double>), pt_grid(), blk(), 0, stream, D.pai, (const double *)D.pf, g);
+      launch(KK_FILTER_BRACKET, fbytes(0, 3), k_sub<double, double>, pt_grid(), blk(), D.pai, (const double *)D.pf, g);
       halo(D.pai, g.nk, 0);
-      hipLaunchKernelGGL(k_smooth5<double>, pt_grid(), blk(), 0, stream, (const double *)D.pai, D.smd, g,
-                         (const double *)nullptr, kNupaitq);
-      hipLaunchKernelGGL((k_addto<double, double>), pt_grid(), blk(), 0, stream, D.pai, (const double *)D.smd,
-                         (const double *)D.pf, g);
+      launch(KK_SMOOTH5, fbytes(0, 2), k_smooth5<double>, pt_grid(), blk(), (const double *)D.pai, D.smd, g,
+             (const double *)nullptr, kNupaitq);
+      launch(KK_FILTER_BRACKET, fbytes(0, 3), k_addto<double, double>, pt_grid(), blk(), D.pai,
+             (const double *)D.smd, (const double *)D.pf, g);
     }
     if (prm.fulleq && prm.filtertheta) {  // :343-353
-      hipLaunchKernelGGL((k_sub<T, T>), pt_grid(), blk(), 0, stream, D.tetav, (const T *)D.tf, g);
+      launch(KK_FILTER_BRACKET, fbytes(3, 0), k_sub<T, T>, pt_grid(), blk(), D.tetav, (const T *)D.tf, g);
       halo(D.tetav, g.nk, 0);
-      hipLaunchKernelGGL(k_smooth5<T>, pt_grid(), blk(), 0, stream, (const T *)D.tetav, D.sm, g,
-                         (const double *)nullptr, kNupaitq);
-      hipLaunchKernelGGL((k_addto<T, T>), pt_grid(), blk(), 0, stream, D.tetav, (const T *)D.sm, (const T *)D.tf, g);
+      launch(KK_SMOOTH5, fbytes(2, 0), k_smooth5<T>, pt_grid(), blk(), (const T *)D.tetav, D.sm, g,
+             (const double *)nullptr, kNupaitq);
+      launch(KK_FILTER_BRACKET, fbytes(3, 0), k_addto<T, T>, pt_grid(), blk(), D.tetav, (const T *)D.sm,
+             (const T *)D.tf, g);
     }
-    hipLaunchKernelGGL(k_epilogue<T>, col_grid(), blk(), 0, stream, D);
+    launch(KK_EPILOGUE, fbytes(11, 1), k_epilogue<T>, col_grid(), blk(), D);
     HIP_TRY(hipGetLastError());
     if (lprint) {
-      hipLaunchKernelGGL(k_diag_partial<T>, col_grid(), blk(), 0, stream, D);
-      hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, stream, D.diag, nblk_diag, d_diag_out);
+      launch(KK_DIAG, fbytes(2, 0), k_diag_partial<T>, col_grid(), blk(), D);
+      launch(KK_DIAG, 0.0, k_diag_final, dim3(1), dim3(256), D.diag, nblk_diag, d_diag_out);
       double out[6];
       HIP_TRY(hipMemcpyAsync(out, d_diag_out, sizeof(out), hipMemcpyDeviceToHost, stream));
       HIP_TRY(hipStreamSynchronize(stream));
@@ -1271,6 +1641,7 @@ extern "C" int mlc_default_params(mlc_params *p) {
   p->filtertheta = 0;
   p->precision = MLC_FP64;
   p->device = 0;
+  p->unfused = 0;
   return MLC_OK;
 }
 
@@ -1356,6 +1727,21 @@ extern "C" int mlc_sync(mlc_handle *h) {
   if (!h) return mlc_fail(MLC_EINVAL, "mlc_sync: NULL handle");
   return h->core->sync();
 }
+
+extern "C" int mlc_profile(mlc_handle *h, int enable) {
+  if (!h) return mlc_fail(MLC_EINVAL, "mlc_profile: NULL handle");
+  return h->core->profile(enable);
+}
+
+extern "C" int mlc_profile_get(mlc_handle *h, int kind, const char **name, double *total_ms, long long *launches,
+                               double *bytes) {
+  if (!h) return mlc_fail(MLC_EINVAL, "mlc_profile_get: NULL handle");
+  if (kind < 0 || kind >= KK_COUNT) return mlc_fail(MLC_EINVAL, "mlc_profile_get: kind out of range");
+  if (name) *name = kKernelNames[kind];
+  return h->core->profile_get(kind, total_ms, launches, bytes);
+}
+
+extern "C" int mlc_profile_kinds(void) { return KK_COUNT; }
 
 extern "C" int mlc_destroy(mlc_handle *h) {
   if (!h) return MLC_OK;

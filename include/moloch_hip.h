@@ -116,6 +116,8 @@ typedef struct mlc_params {
                     /* module parameters do_* (nompi/mod_moloch.F90:100-104) */
   int precision;    /* MLC_FP64 or MLC_FP32                               */
   int device;       /* HIP device ordinal                                 */
+  int unfused;      /* 1: one kernel per phase as in the reference's loop */
+                    /* nests (for comparisons); 0: fused kernels (default) */
 } mlc_params;
 
 /* lprint diagnostics of nompi/mod_moloch.F90:395-419 */
@@ -162,6 +164,19 @@ int mlc_step(mlc_handle *h, int istep, int lprint, mlc_diag *diag);
 
 /* Wait for all work queued on the handle's stream. */
 int mlc_sync(mlc_handle *h);
+
+/* Per-kernel timing for benchmarks (no counterpart in the reference, whose
+ * only timer is whole-run cpu_time, nompi/regcm_dynamical_core.F90:30,222-228).
+ * mlc_profile(h, 1) brackets every subsequent kernel launch of the handle
+ * with HIP events on its stream and clears the accumulators; mlc_profile(h, 0)
+ * stops.  mlc_profile_get returns, for kernel kind 0 <= kind <
+ * mlc_profile_kinds(), its name, the summed event time, the number of
+ * launches and the summed algorithmic bytes (the fields the kernel must read
+ * and write once each, kz levels x jx*iy columns). */
+int mlc_profile(mlc_handle *h, int enable);
+int mlc_profile_kinds(void);
+int mlc_profile_get(mlc_handle *h, int kind, const char **name, double *total_ms, long long *launches,
+                    double *bytes);
 
 /* Release every device allocation of the handle. */
 int mlc_destroy(mlc_handle *h);
