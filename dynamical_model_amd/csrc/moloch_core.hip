@@ -91,7 +91,6 @@ struct Dev {
   const double *gzitak, *gzitakh, *xknu, *ffilt;
   // scratch
   T *s, *deltaw, *zdh, *zds, *zdf, *wx, *qwltot, *qwitot, *tf, *sm;
-  T *spare;               // output buffer of the fused advection (swapped in)
   double *wz, *p0;        // advection scratch, sized for an FP64 field
   double *pf, *smd;       // filter scratch
   double *thg, *thc;      // tridiagonal elimination (FP64)
@@ -718,36 +717,6 @@ __global__ void __launch_bounds__(256) k_waf_z(Dev<T> D, Par P, Q *pp, const Q *
   pp[x] = out;
 }
 
-// ========================================================== fused wafone
-// One launch advects one field through the two vertical half passes, the
-// meridional and the zonal sweep (:929-1311 with do_vadvtwice).  A block
-// owns a tile of WJ x WI columns and marches down the levels.  Its threads
-// run the vertical passes as a register pipeline on the tile extended by two
-// columns on every side (the reach of the horizontal sweeps), so output
-// level k is ready when input level k+4 arrives; each ready level goes
-// through LDS planes -- vertical result, original field and fmz -- from
-// which the meridional sweep builds p0 on the rows of the extended tile and
-// the zonal sweep the result on the tile.  Halo columns are the periodic
-// images, i.e. the values the reference's halo copies hold (:1075-1078,
-// :1133-1136).  Arithmetic and operand order are those of the unfused
-// kernels.  The result goes to a second buffer (pp_out).
-constexpr int WJ = 28, WI = 12;            // interior tile
-constexpr int EJ = WJ + 4, EI = WI + 4;    // extended tile (32 x 16)
-constexpr int WTHREADS = 256;              // 32 x 8 threads, 2 columns each
-
-// Per-column state of the vertical pipeline.  Windows are indexed by level
-// relative to the input level m of the current iteration: slot 4 = level m,
-// slot 0 = level m-4 (P, S, ZU, ZD, FZ); W holds first-pass values of
-// levels m-5..m-2 (slot 3 = m-2).  Levels outside 1..nk hold the clamped
-// values the reference's k1/k1p1 clamps select (:980-990).
-template <typename Q>
-struct VPipe {
-  Q P[5], S[5], ZU[5], ZD[5], FZ[5];
-  Q W[4];
-  Q F1, F2;   // flux at the upper interface of the level the pass finishes next
-  Q ffz;      // fmzf of level m
-};
-
 // WAF flux through interface m (between levels m-1 above and m below),
 // :966-1001 written for a window: qm2..q1 are levels m-2..m+1, clamped.
 template <typename Q>
@@ -769,200 +738,331 @@ __device__ __forceinline__ Q waf_vflux(Q s, Q dtrdz, Q qm2, Q qm1, Q q0, Q q1) {
   return Q(0.5) * s * ((Q(1) + zphi) * q0 + (Q(1) - zphi) * qm1);
 }
 
-template <typename T, typename Q>
-__global__ void __launch_bounds__(WTHREADS) k_waf_fused(Dev<T> D, Par P, const Q *__restrict__ pp,
-                                                         Q *__restrict__ out, double dta) {
-  __shared__ Q sW[2][EI][EJ];
-  __shared__ Q sP[2][EI][EJ];
-  __shared__ Q sF[2][EI][EJ];
-  __shared__ Q sP0[2][WI][EJ];
-  const Geo &g = D.g;
-  const int nj = g.nj, ni = g.ni, nk = g.nk;
-  const int tx = threadIdx.x % EJ, ty = threadIdx.x / EJ;  // ty in 0..7
-  const int j0 = blockIdx.x * WJ + 1, i0 = blockIdx.y * WI + 1;
+// ============================================= multi-field wafone, 2 kernels
+// The advected set (:911-920) goes through wafone in groups of up to MAXF
+// fields of one storage type.  k_wafv runs both vertical half passes for
+// every field of the group, one column per lane, sharing the loads of s,
+// fmz, fmzf and the zrfmu/zrfmd divisions between the fields.  k_wafh then
+// does the meridional and zonal sweeps of every field of the group for one
+// (tile, level) per block, reading u, v, fmz once into LDS.  Same arithmetic
+// as the single-field kernels.
+constexpr int MAXF = 5;
+
+template <typename Q>
+struct FieldSet {
+  const Q *in[MAXF];  // fields before advection (also read by the zdv terms)
+  Q *wz[MAXF];        // after the vertical passes
+  Q *out[MAXF];       // after the horizontal sweeps
+};
+
+template <typename T, typename Q, int NF>
+__global__ void __launch_bounds__(256) k_wafv(Dev<T> D, Par P, FieldSet<Q> F, double dta) {
+  COL_IDX
+  const int nk = g.nk;
   const Q dtrdz = Q(0.5) * (Q)(dta / P.dzita);
-  const Q dtrdx = (Q)(dta / P.dx), dtrdy = (Q)(dta / P.dx);
-  const int imin = -1, imax = ni + 2, jmin = -1, jmax = nj + 2;  // :264-267
-
-  long cb[2];
-  VPipe<Q> V[2];
-  for (int c = 0; c < 2; c++) {
-    int jw = ((j0 - 2 + tx - 1) % nj + nj) % nj + 1;
-    int iw = ((i0 - 2 + ty + 8 * c - 1) % ni + ni) % ni + 1;
-    cb[c] = g.at(jw, iw, 1);
-  }
-
+  const long c0 = g.at(j, i, 1);
+  const long pl = g.plane;
+  // windows: S interfaces m-4..m, ZU/ZD levels m-4..m (slot 4 = m);
+  // Pw levels m-3..m (slot 3 = m); Ww first-pass levels m-5..m-2
+  Q S[5], ZU[5], ZD[5], ffz;
+  Q Pw[NF][4], Ww[NF][4], F1[NF], F2[NF];
   for (int m = 1; m <= nk + 4; m++) {
+    const int mm = m <= nk ? m : nk;
+    const long xm = c0 + (long)(mm - 1) * pl;
+    const Q sm = m <= nk + 1 ? (Q)D.s[c0 + (long)(m - 1) * pl] : Q(0);
+    const Q fzm = (Q)D.fmz[xm];
+    if (m == 1) {
+      ffz = (Q)D.fmzf[c0];
+#pragma unroll
+      for (int q = 0; q < 5; q++) S[q] = Q(0), ZU[q] = Q(0), ZD[q] = Q(0);
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        Q p1 = F.in[f][xm];
+#pragma unroll
+        for (int q = 0; q < 4; q++) Pw[f][q] = p1;
+        F1[f] = Q(0);
+        F2[f] = Q(0);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) S[q] = S[q + 1], ZU[q] = ZU[q + 1], ZD[q] = ZD[q + 1];
+    S[4] = sm;
+    if (m <= nk) {
+      Q ff1 = (Q)D.fmzf[xm + pl];
+      ZU[4] = dtrdz * fzm / ffz;  // zrfmu (:1010)
+      ZD[4] = dtrdz * fzm / ff1;  // zrfmd (:1011)
+      ffz = ff1;
+    }
     const int kb = m - 2, kc = m - 4;
 #pragma unroll
-    for (int c = 0; c < 2; c++) {
-      VPipe<Q> &v = V[c];
-      const long c0 = cb[c];
-      // ---- receive level m (clamped to nk) and interface m
-      const int mm = m <= nk ? m : nk;
-      const long xm = c0 + (long)(mm - 1) * g.plane;
-      Q pm = pp[xm];
-      Q fzm = (Q)D.fmz[xm];
-      Q sm = m <= nk + 1 ? (Q)D.s[c0 + (long)(m - 1) * g.plane] : Q(0);
-      if (m == 1) {
-        v.ffz = (Q)D.fmzf[c0];
+    for (int f = 0; f < NF; f++) {
+      Q pm = F.in[f][xm];
 #pragma unroll
-        for (int q = 0; q < 5; q++) v.P[q] = pm, v.S[q] = Q(0), v.FZ[q] = fzm, v.ZU[q] = Q(0), v.ZD[q] = Q(0);
-        v.F1 = Q(0);
-        v.F2 = Q(0);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        v.P[q] = v.P[q + 1];
-        v.S[q] = v.S[q + 1];
-        v.FZ[q] = v.FZ[q + 1];
-        v.ZU[q] = v.ZU[q + 1];
-        v.ZD[q] = v.ZD[q + 1];
-      }
-      v.P[4] = pm;
-      v.S[4] = sm;
-      v.FZ[4] = fzm;
-      if (m <= nk) {
-        Q ff1 = (Q)D.fmzf[xm + g.plane];
-        v.ZU[4] = dtrdz * fzm / v.ffz;  // zrfmu (:1010)
-        v.ZD[4] = dtrdz * fzm / ff1;    // zrfmd (:1011)
-        v.ffz = ff1;
-      }
-      // ---- first pass: level kb = m-2 from interfaces m-2 (F1) and m-1
+      for (int q = 0; q < 3; q++) Pw[f][q] = Pw[f][q + 1];
+      Pw[f][3] = pm;
+      // first pass, level kb from interfaces kb (F1) and kb+1 = m-1
       if (kb >= 1 && kb <= nk) {
         Q f1 = Q(0);
-        if (m - 1 >= 2 && m - 1 <= nk) f1 = waf_vflux(v.S[3], dtrdz, v.P[1], v.P[2], v.P[3], v.P[4]);
-        Q q = v.P[2];
-        Q zdv = (v.S[2] * v.ZU[2] - v.S[3] * v.ZD[2]) * q;
-        Q w1 = q - v.F1 * v.ZU[2] + f1 * v.ZD[2] + zdv;
-        v.F1 = f1;
+        if (m - 1 >= 2 && m - 1 <= nk) f1 = waf_vflux(S[3], dtrdz, Pw[f][0], Pw[f][1], Pw[f][2], Pw[f][3]);
+        Q q = Pw[f][1];
+        Q zdv = (S[2] * ZU[2] - S[3] * ZD[2]) * q;
+        Q w1 = q - F1[f] * ZU[2] + f1 * ZD[2] + zdv;
+        F1[f] = f1;
         if (kb == 1) {
-          v.W[0] = v.W[1] = v.W[2] = v.W[3] = w1;
+          Ww[f][0] = Ww[f][1] = Ww[f][2] = Ww[f][3] = w1;
         } else {
-          v.W[0] = v.W[1];
-          v.W[1] = v.W[2];
-          v.W[2] = v.W[3];
-          v.W[3] = w1;
+          Ww[f][0] = Ww[f][1];
+          Ww[f][1] = Ww[f][2];
+          Ww[f][2] = Ww[f][3];
+          Ww[f][3] = w1;
         }
       } else if (kb > nk) {
-        v.W[0] = v.W[1];
-        v.W[1] = v.W[2];
-        v.W[2] = v.W[3];  // level kb clamps to nk
+        Ww[f][0] = Ww[f][1];
+        Ww[f][1] = Ww[f][2];
+        Ww[f][2] = Ww[f][3];
       }
-      // ---- second pass: level kc = m-4 from interfaces m-4 (F2) and m-3
+      // second pass, level kc from interfaces kc (F2) and kc+1 = m-3
       if (kc >= 1 && kc <= nk) {
         Q f2 = Q(0);
-        if (m - 3 >= 2 && m - 3 <= nk) f2 = waf_vflux(v.S[1], dtrdz, v.W[0], v.W[1], v.W[2], v.W[3]);
-        Q q = v.W[1];
-        Q zdv = (v.S[0] * v.ZU[0] - v.S[1] * v.ZD[0]) * q;
-        Q w2 = q - v.F2 * v.ZU[0] + f2 * v.ZD[0] + zdv;
-        v.F2 = f2;
-        const int ei = ty + 8 * c, b = kc & 1;
-        sW[b][ei][tx] = w2;
-        sP[b][ei][tx] = v.P[0];
-        sF[b][ei][tx] = v.FZ[0];
+        if (m - 3 >= 2 && m - 3 <= nk) f2 = waf_vflux(S[1], dtrdz, Ww[f][0], Ww[f][1], Ww[f][2], Ww[f][3]);
+        Q q = Ww[f][1];
+        Q zdv = (S[0] * ZU[0] - S[1] * ZD[0]) * q;
+        F.wz[f][c0 + (long)(kc - 1) * pl] = q - F2[f] * ZU[0] + f2 * ZD[0] + zdv;
+        F2[f] = f2;
       }
     }
-    if (kc < 1) continue;  // uniform across the block
-    const int b = kc & 1;
+  }
+}
+
+// Horizontal sweeps of one level on a tile of HJ x HI cells.
+constexpr int HJ = 60, HI = 12;
+constexpr int HEJ = HJ + 4, HEI = HI + 4;  // 64 x 16 with the reach of both sweeps
+
+// Block index -> tile, grouping consecutive tiles on one XCD: the hardware
+// hands consecutive workgroups to the eight XCDs in turn, and neighbouring
+// tiles share halo lines through that XCD's L2.
+__device__ __forceinline__ long xcd_remap(long b, long total) {
+  const long nx = 8;
+  long chunk = (total + nx - 1) / nx;
+  return (b % nx) * chunk + b / nx;
+}
+
+// One (tile, level) per block; for each field of the group: meridional face
+// fluxes -> p0 on the rows of the extended tile -> zonal face fluxes ->
+// result.  Everything that does not depend on the advected field (Courant
+// numbers, fmz ratios, the divergence-correction coefficients) is evaluated
+// once per block; each face flux once per field.
+template <typename T, typename Q, int NF>
+__global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, double dta, int ntj, int nti) {
+  __shared__ Q sWz[HEI][HEJ];       // wz, rows i0-2 .. i0+HI+1
+  __shared__ Q sPp[2][HI][HEJ];     // field, rows i0 .. i0+HI-1 (double buffered)
+  __shared__ Q sZp[HI + 1][HEJ];    // meridional flux at the south face of rows i0 .. i0+HI
+  __shared__ Q sP0[HI][HEJ];        // p0
+  __shared__ Q sZw[HI][HJ + 1];     // zonal flux at the west face of columns j0 .. j0+HJ
+  const Geo &g = D.g;
+  const int nj = g.nj, ni = g.ni;
+  const long total = (long)ntj * nti * g.nk;
+  const long lb = xcd_remap((long)blockIdx.x, total);
+  if (lb >= total) return;
+  const int tj = (int)(lb % ntj), ti = (int)((lb / ntj) % nti), k = (int)(lb / ((long)ntj * nti)) + 1;
+  const int j0 = tj * HJ + 1, i0 = ti * HI + 1;
+  const int tid = threadIdx.x, ej = tid % HEJ, rr = tid / HEJ;
+  const Q dtrdx = (Q)(dta / P.dx), dtrdy = (Q)(dta / P.dx);
+  auto wj = [&](int j) { return j < 1 ? j + nj : (j > nj ? j - nj : j); };
+  auto wi = [&](int i) { return i < 1 ? i + ni : (i > ni ? i - ni : i); };
+  const int jw = wj(j0 - 2 + ej);
+
+  // ---- field-independent parts
+  // meridional faces f = rr + 4n (south face of row i0+f), n = 0..3
+  Q mz[4], ma[4];
+#pragma unroll
+  for (int n = 0; n < 4; n++) {
+    const int f = rr + 4 * n;
+    mz[n] = Q(0);
+    ma[n] = Q(0);
+    if (f <= HI) {
+      const int iw = wi(i0 + f);
+      Q vv = (Q)D.v[g.at(jw, iw, k)];
+      if (P.rotllr) {
+        mz[n] = vv * dtrdy;
+        ma[n] = Q(0.5) * vv;
+      } else {
+        Q rm = (Q)D.rmv[g.at2(jw, iw)];
+        mz[n] = vv * rm * dtrdy;
+        ma[n] = Q(0.5) * vv * rm;
+      }
+    }
+  }
+  // p0 points r = rr + 4n, n = 0..2
+  Q pfs[3], pfn[3], pcd[3], pm2[3];
+#pragma unroll
+  for (int n = 0; n < 3; n++) {
+    const int r = rr + 4 * n;
+    const int iw = wi(i0 + r), ip = wi(i0 + r + 1), im = wi(i0 + r - 1);
+    Q f0 = (Q)D.fmz[g.at(jw, iw, k)], fn = (Q)D.fmz[g.at(jw, ip, k)], fs = (Q)D.fmz[g.at(jw, im, k)];
+    Q vc = (Q)D.v[g.at(jw, iw, k)], vp = (Q)D.v[g.at(jw, ip, k)];
+    if (P.rotllr) {
+      Q mx = (Q)D.mx[g.at2(jw, iw)];
+      Q zhxvtn = dtrdy * (Q)D.rmv[g.at2(jw, ip)] * mx;
+      Q zhxvts = dtrdy * (Q)D.rmv[g.at2(jw, iw)] * mx;
+      pfn[n] = zhxvtn * Q(2) * f0 / (f0 + fn);
+      pfs[n] = zhxvts * Q(2) * f0 / (f0 + fs);
+      pcd[n] = vp * pfn[n] - vc * pfs[n];
+      pm2[n] = Q(1);
+    } else {
+      pfn[n] = dtrdy * Q(2) * f0 / (f0 + fn);
+      pfs[n] = dtrdy * Q(2) * f0 / (f0 + fs);
+      pcd[n] = vp * (Q)D.rmv[g.at2(jw, ip)] * pfn[n] - vc * (Q)D.rmv[g.at2(jw, iw)] * pfs[n];
+      pm2[n] = (Q)D.mx2[g.at2(jw, iw)];
+    }
+  }
+  // zonal faces q = tid + 256n over (jf, r) in (HJ+1) x HI, n = 0..2
+  Q zz[3], za[3];
+#pragma unroll
+  for (int n = 0; n < 3; n++) {
+    const int q = tid + 256 * n;
+    zz[n] = Q(0);
+    za[n] = Q(0);
+    if (q < (HJ + 1) * HI) {
+      const int jf = q % (HJ + 1), r = q / (HJ + 1);
+      const int jw2 = wj(j0 + jf), iw = wi(i0 + r);
+      Q uu = (Q)D.u[g.at(jw2, iw, k)];
+      if (P.rotllr) {
+        zz[n] = uu * (Q)D.mu[g.at2(jw2, iw)] * dtrdx;
+        za[n] = Q(0.5) * uu;
+      } else {
+        Q rm = (Q)D.rmu[g.at2(jw2, iw)];
+        zz[n] = uu * rm * dtrdx;
+        za[n] = Q(0.5) * uu * rm;
+      }
+    }
+  }
+  // zonal points q = tid + 256n over (jt, r) in HJ x HI, n = 0..2
+  Q zfw[3], zfe[3], zcd[3], zm2[3];
+#pragma unroll
+  for (int n = 0; n < 3; n++) {
+    const int q = tid + 256 * n;
+    zfw[n] = zfe[n] = zcd[n] = zm2[n] = Q(0);
+    if (q < HJ * HI) {
+      const int jt = q % HJ, r = q / HJ;
+      const int j = wj(j0 + jt), jp = wj(j0 + jt + 1), jm = wj(j0 + jt - 1), iw = wi(i0 + r);
+      Q f0 = (Q)D.fmz[g.at(j, iw, k)], fe = (Q)D.fmz[g.at(jp, iw, k)], fw = (Q)D.fmz[g.at(jm, iw, k)];
+      Q uc = (Q)D.u[g.at(j, iw, k)], up = (Q)D.u[g.at(jp, iw, k)];
+      if (P.rotllr) {
+        Q zcostx = dtrdx * (Q)D.mu[g.at2(j, iw)];
+        zfe[n] = zcostx * Q(2) * f0 / (f0 + fe);
+        zfw[n] = zcostx * Q(2) * f0 / (f0 + fw);
+        zcd[n] = up * zfe[n] - uc * zfw[n];
+        zm2[n] = Q(1);
+      } else {
+        zfe[n] = dtrdx * Q(2) * f0 / (f0 + fe);
+        zfw[n] = dtrdx * Q(2) * f0 / (f0 + fw);
+        zcd[n] = up * (Q)D.rmu[g.at2(jp, iw)] * zfe[n] - uc * (Q)D.rmu[g.at2(j, iw)] * zfw[n];
+        zm2[n] = (Q)D.mx2[g.at2(j, iw)];
+      }
+    }
+  }
+
+  // ---- field loop, loads of the next field issued ahead
+  Q lw[4], lp[3];
+  auto fetch = [&](int f) {
+#pragma unroll
+    for (int n = 0; n < 4; n++) lw[n] = F.wz[f][g.at(jw, wi(i0 - 2 + rr + 4 * n), k)];
+#pragma unroll
+    for (int n = 0; n < 3; n++) lp[n] = F.in[f][g.at(jw, wi(i0 + rr + 4 * n), k)];
+  };
+  fetch(0);
+  for (int f = 0; f < NF; f++) {
+    const int pb = f & 1;
+#pragma unroll
+    for (int n = 0; n < 4; n++) sWz[rr + 4 * n][ej] = lw[n];
+#pragma unroll
+    for (int n = 0; n < 3; n++) sPp[pb][rr + 4 * n][ej] = lp[n];
     __syncthreads();
-    // ---- meridional sweep on the rows of the extended tile -> p0
-    for (int q = threadIdx.x; q < EJ * WI; q += WTHREADS) {
-      const int ej = q % EJ, it = q / EJ;
-      const int i = i0 + it;
-      if (i > ni) continue;
-      const int jw = ((j0 - 2 + ej - 1) % nj + nj) % nj + 1;
-      const int ec = it + 2;  // extended row of cell i
-      auto fl = [&](int ii, int er) -> Q {  // flux at south face of cell ii (extended row er)
-        int iw2 = ii > ni ? ii - ni : ii;
-        Q vv = (Q)D.v[g.at(jw, iw2, kc)];
-        Q zamu = P.rotllr ? vv * dtrdy : vv * (Q)D.rmv[g.at2(jw, iw2)] * dtrdy;
-        Q is;
-        int ih;
+    if (f + 1 < NF) fetch(f + 1);
+    // meridional face fluxes (:1082-1100)
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+      const int fr = rr + 4 * n;
+      if (fr <= HI) {
+        Q zamu = mz[n], is;
+        int oh, ohm1;  // rows of ih and ihm1 relative to the face's cell row
         if (zamu > Q(0)) {
           is = Q(1);
-          ih = ii - 1;
+          oh = -1;
+          ohm1 = -2;
         } else {
           is = Q(-1);
-          ih = ii + 1 < imax ? ii + 1 : imax;
+          oh = 1;
+          ohm1 = 0;
         }
-        int ihm1 = ih - 1 > imin ? ih - 1 : imin;
-        Q wc = sW[b][er][ej], wm = sW[b][er - 1][ej];
+        const int er = fr + 2;  // row of the cell north of the face
+        Q wc = sWz[er][ej], wm = sWz[er - 1][ej];
         Q zzden = sgn_floor(wc - wm);
-        Q r = (sW[b][er + (ih - ii)][ej] - sW[b][er + (ihm1 - ii)][ej]) / zzden;
-        Q bb = waf_b(r);
+        Q rq = (sWz[er + oh][ej] - sWz[er + ohm1][ej]) / zzden;
+        Q bb = waf_b(rq);
         Q zphi = is + zamu * bb - is * bb;
-        if (P.rotllr) return Q(0.5) * vv * ((Q(1) + zphi) * wm + (Q(1) - zphi) * wc);
-        return Q(0.5) * vv * (Q)D.rmv[g.at2(jw, iw2)] * ((Q(1) + zphi) * wm + (Q(1) - zphi) * wc);
-      };
-      Q zs = fl(i, ec), zn = fl(i + 1, ec + 1);
-      Q f0 = sF[b][ec][ej];
-      const int ip = i + 1 > ni ? 1 : i + 1;
-      Q vc = (Q)D.v[g.at(jw, i, kc)], vp = (Q)D.v[g.at(jw, ip, kc)];
-      Q o;
-      if (P.rotllr) {
-        Q mx = (Q)D.mx[g.at2(jw, i)];
-        Q zhxvtn = dtrdy * (Q)D.rmv[g.at2(jw, ip)] * mx;
-        Q zhxvts = dtrdy * (Q)D.rmv[g.at2(jw, i)] * mx;
-        Q zrfmn = zhxvtn * Q(2) * f0 / (f0 + sF[b][ec + 1][ej]);
-        Q zrfms = zhxvts * Q(2) * f0 / (f0 + sF[b][ec - 1][ej]);
-        Q zdv = (vp * zrfmn - vc * zrfms) * sP[b][ec][ej];
-        o = sW[b][ec][ej] + zs * zrfms - zn * zrfmn + zdv;
-      } else {
-        Q zrfmn = dtrdy * Q(2) * f0 / (f0 + sF[b][ec + 1][ej]);
-        Q zrfms = dtrdy * Q(2) * f0 / (f0 + sF[b][ec - 1][ej]);
-        Q zdv = (vp * (Q)D.rmv[g.at2(jw, ip)] * zrfmn - vc * (Q)D.rmv[g.at2(jw, i)] * zrfms) * sP[b][ec][ej];
-        o = sW[b][ec][ej] + (Q)D.mx2[g.at2(jw, i)] * (zs * zrfms - zn * zrfmn + zdv);
+        sZp[fr][ej] = ma[n] * ((Q(1) + zphi) * wm + (Q(1) - zphi) * wc);
       }
-      sP0[b][it][ej] = o;
     }
     __syncthreads();
-    // ---- zonal sweep on the tile -> out
-    for (int q = threadIdx.x; q < WJ * WI; q += WTHREADS) {
-      const int jt = q % WJ, it = q / WJ;
-      const int j = j0 + jt, i = i0 + it;
-      if (j > nj || i > ni) continue;
-      const int ec = jt + 2, er = it + 2;
-      auto fl = [&](int jj, int e) -> Q {  // flux at west face of cell jj (extended column e)
-        int jw2 = jj > nj ? jj - nj : jj;
-        Q uu = (Q)D.u[g.at(jw2, i, kc)];
-        Q zamu = P.rotllr ? uu * (Q)D.mu[g.at2(jw2, i)] * dtrdx : uu * (Q)D.rmu[g.at2(jw2, i)] * dtrdx;
-        Q is;
-        int jh;
+    // p0 (:1104-1131)
+#pragma unroll
+    for (int n = 0; n < 3; n++) {
+      const int r = rr + 4 * n;
+      Q zdv = pcd[n] * sPp[pb][r][ej];
+      Q o;
+      if (P.rotllr)
+        o = sWz[r + 2][ej] + sZp[r][ej] * pfs[n] - sZp[r + 1][ej] * pfn[n] + zdv;
+      else
+        o = sWz[r + 2][ej] + pm2[n] * (sZp[r][ej] * pfs[n] - sZp[r + 1][ej] * pfn[n] + zdv);
+      sP0[r][ej] = o;
+    }
+    __syncthreads();
+    // zonal face fluxes (:1138-1160)
+#pragma unroll
+    for (int n = 0; n < 3; n++) {
+      const int q = tid + 256 * n;
+      if (q < (HJ + 1) * HI) {
+        const int jf = q % (HJ + 1), r = q / (HJ + 1);
+        Q zamu = zz[n], is;
+        int oh, ohm1;
         if (zamu > Q(0)) {
           is = Q(1);
-          jh = jj - 1;
+          oh = -1;
+          ohm1 = -2;
         } else {
           is = Q(-1);
-          jh = jj + 1 < jmax ? jj + 1 : jmax;
+          oh = 1;
+          ohm1 = 0;
         }
-        int jhm1 = jh - 1 > jmin ? jh - 1 : jmin;
-        Q pc = sP0[b][it][e], pm = sP0[b][it][e - 1];
+        const int e = jf + 2;  // extended column of the cell east of the face
+        Q pc = sP0[r][e], pm = sP0[r][e - 1];
         Q zzden = sgn_floor(pc - pm);
-        Q r = (sP0[b][it][e + (jh - jj)] - sP0[b][it][e + (jhm1 - jj)]) / zzden;
-        Q bb = waf_b(r);
+        Q rq = (sP0[r][e + oh] - sP0[r][e + ohm1]) / zzden;
+        Q bb = waf_b(rq);
         Q zphi = is + zamu * bb - is * bb;
-        if (P.rotllr) return Q(0.5) * uu * ((Q(1) + zphi) * pm + (Q(1) - zphi) * pc);
-        return Q(0.5) * uu * (Q)D.rmu[g.at2(jw2, i)] * ((Q(1) + zphi) * pm + (Q(1) - zphi) * pc);
-      };
-      Q zw = fl(j, ec), ze = fl(j + 1, ec + 1);
-      Q f0 = sF[b][er][ec];
-      const int jp = j + 1 > nj ? 1 : j + 1;
-      Q uc = (Q)D.u[g.at(j, i, kc)], up = (Q)D.u[g.at(jp, i, kc)];
-      Q o;
-      if (P.rotllr) {
-        Q zcostx = dtrdx * (Q)D.mu[g.at2(j, i)];
-        Q zrfme = zcostx * Q(2) * f0 / (f0 + sF[b][er][ec + 1]);
-        Q zrfmw = zcostx * Q(2) * f0 / (f0 + sF[b][er][ec - 1]);
-        Q zdv = (up * zrfme - uc * zrfmw) * sP[b][er][ec];
-        o = sP0[b][it][ec] + zw * zrfmw - ze * zrfme + zdv;
-      } else {
-        Q zrfme = dtrdx * Q(2) * f0 / (f0 + sF[b][er][ec + 1]);
-        Q zrfmw = dtrdx * Q(2) * f0 / (f0 + sF[b][er][ec - 1]);
-        Q zdv = (up * (Q)D.rmu[g.at2(jp, i)] * zrfme - uc * (Q)D.rmu[g.at2(j, i)] * zrfmw) * sP[b][er][ec];
-        o = sP0[b][it][ec] + (Q)D.mx2[g.at2(j, i)] * (zw * zrfmw - ze * zrfme + zdv);
+        sZw[r][jf] = za[n] * ((Q(1) + zphi) * pm + (Q(1) - zphi) * pc);
       }
-      out[g.at(j, i, kc)] = o;
+    }
+    __syncthreads();
+    // result (:1162-1187)
+#pragma unroll
+    for (int n = 0; n < 3; n++) {
+      const int q = tid + 256 * n;
+      if (q < HJ * HI) {
+        const int jt = q % HJ, r = q / HJ;
+        const int j = j0 + jt, i = i0 + r;
+        if (j <= nj && i <= ni) {
+          Q zdv = zcd[n] * sPp[pb][r][jt + 2];
+          Q o;
+          if (P.rotllr)
+            o = sP0[r][jt + 2] + sZw[r][jt] * zfw[n] - sZw[r][jt + 1] * zfe[n] + zdv;
+          else
+            o = sP0[r][jt + 2] + zm2[n] * (sZw[r][jt] * zfw[n] - sZw[r][jt + 1] * zfe[n] + zdv);
+          F.out[f][g.at(j, i, k)] = o;
+        }
+      }
     }
   }
 }
@@ -1100,11 +1200,11 @@ __global__ void __launch_bounds__(256) k_diag_final(double *partial, long nblk, 
 // Kernel kinds for the per-launch event timing of mlc_profile_*.
 enum KernelKind {
   KK_HALO = 0, KK_PROLOGUE, KK_SOUND_A, KK_SOUND_B, KK_SOUND_B2, KK_SOUND_C, KK_SMOOTH5, KK_FILTER_BRACKET,
-  KK_ADV_IN, KK_WAF_V, KK_WAF_M, KK_WAF_Z, KK_WAF_FUSED, KK_ADV_OUT, KK_EPILOGUE, KK_DIAG, KK_COUNT
+  KK_ADV_IN, KK_WAF_V, KK_WAF_M, KK_WAF_Z, KK_WAF_H, KK_ADV_OUT, KK_EPILOGUE, KK_DIAG, KK_COUNT
 };
 static const char *kKernelNames[KK_COUNT] = {
     "halo", "prologue", "sound_a", "sound_b", "sound_b2", "sound_c", "smooth5", "filter_bracket",
-    "adv_in", "waf_v", "waf_m", "waf_z", "waf_fused", "adv_out", "epilogue", "diag"};
+    "adv_in", "waf_v", "waf_m", "waf_z", "waf_h", "adv_out", "epilogue", "diag"};
 
 struct CoreBase {
   virtual ~CoreBase() {}
@@ -1131,6 +1231,7 @@ struct Core : CoreBase {
   std::vector<double> h_msfu, h_msfv, h_msfx, h_ulat, h_vlat;  // host geometry
   bool initialised = false;
   bool unfused_advection = false;  // params.unfused: the three-sweep wafone kernels
+  T *gwz[MAXF] = {}, *gout[MAXF] = {};  // scratch of the grouped advection
   long nblk_diag = 0;
   double *d_diag_out = nullptr;
 
@@ -1253,7 +1354,11 @@ struct Core : CoreBase {
     AF(fmz, K) AF(fmzf, K1) AF(zeta, K) AF(hx, 1) AF(hy, 1) AF(mu, 1) AF(mv, 1) AF(mx, 1);
     AF(rmu, 1) AF(rmv, 1) AF(mx2, 1) AF(coru, 1) AF(corv, 1);
     AF(s, K1) AF(deltaw, K1) AF(zdh, K) AF(zds, K) AF(zdf, K) AF(wx, K) AF(qwltot, K) AF(qwitot, K);
-    AF(tf, K) AF(sm, K) AF(spare, K);
+    AF(tf, K) AF(sm, K);
+    for (int f = 0; f < MAXF; f++) {
+      if ((rc = alloc_field(&gwz[f], K)) != MLC_OK) return rc;
+      if ((rc = alloc_field(&gout[f], K)) != MLC_OK) return rc;
+    }
 #undef AF
     if ((rc = alloc_d(&D.pai, g.plane * K)) != MLC_OK) return rc;
     if ((rc = alloc_d(&D.wz, g.plane * K)) != MLC_OK) return rc;
@@ -1510,21 +1615,12 @@ struct Core : CoreBase {
     }
   }
 
-  // wafone (:929-1311) on the field pointed to by pp.  With the two vertical
-  // half passes the fused kernel writes the result to the spare buffer of
-  // the field's type, which then takes the field's place.
+  // wafone (:929-1311) on one field with the three sweep kernels (the
+  // single vertical pass, or params.unfused).
   template <typename Q>
-  void wafone(Q *&pp, Q *&spare, double dta) {
+  void wafone(Q *pp, double dta) {
     Par P = par();
     const double q = sizeof(Q) == sizeof(double) ? 1.0 : 0.0;  // advected field in FP64?
-    if (prm.vadvtwice && !unfused_advection) {
-      // pp, s, fmz, fmzf, u, v -> result
-      dim3 grid((g.nj + WJ - 1) / WJ, (g.ni + WI - 1) / WI, 1);
-      launch(KK_WAF_FUSED, fbytes(4 + 2 * (1 - q), 2 * q), k_waf_fused<T, Q>, grid, dim3(WTHREADS), D, P,
-             (const Q *)pp, spare, dta);
-      std::swap(pp, spare);
-      return;
-    }
     Q *wz = (Q *)D.wz, *p0 = (Q *)D.p0;
     // vertical: pp, s, fmz, fmzf -> wz;  meridional: wz, v, fmz, pp -> p0;
     // zonal: p0, u, fmz, pp -> pp
@@ -1535,18 +1631,81 @@ struct Core : CoreBase {
     launch(KK_WAF_Z, fbytes(2 + 3 * (1 - q), 3 * q), k_waf_z<T, Q>, pt_grid(), blk(), D, P, pp, (const Q *)p0, dta);
   }
 
+  // wafone on a group of fields of one storage type: k_wafv then k_wafh; each
+  // field's result buffer then takes the field's place.
+  template <typename Q, int NF>
+  void waf_group(Q **fld[], Q **outs, Q **wzs, double dta) {
+    Par P = par();
+    FieldSet<Q> F;
+    for (int f = 0; f < MAXF; f++) F.in[f] = nullptr, F.wz[f] = nullptr, F.out[f] = nullptr;
+    for (int f = 0; f < NF; f++) {
+      F.in[f] = *fld[f];
+      F.wz[f] = wzs[f];
+      F.out[f] = outs[f];
+    }
+    const double q = sizeof(Q) == sizeof(double) ? 1.0 : 0.0;
+    const double sT = sizeof(T) == sizeof(double) ? 1.0 : 0.0;
+    // vertical: s, fmz, fmzf + NF fields -> NF wz
+    launch(KK_WAF_V, fbytes(3, 0) + (q ? fbytes(0, 2 * NF) : fbytes(2 * NF, 0)), k_wafv<T, Q, NF>, col_grid(),
+           blk(), D, P, F, dta);
+    // horizontal: u, v, fmz + NF x (wz, field) -> NF results
+    const int ntj = (g.nj + HJ - 1) / HJ, nti = (g.ni + HI - 1) / HI;
+    const long total = (long)ntj * nti * g.nk;
+    launch(KK_WAF_H, fbytes(3, 0) + (q ? fbytes(0, 3 * NF) : fbytes(3 * NF, 0)), k_wafh<T, Q, NF>,
+           dim3((unsigned)round_up(total, 8)), dim3(256), D, P, F, dta, ntj, nti);
+    (void)sT;
+    for (int f = 0; f < NF; f++) std::swap(*fld[f], outs[f]);
+  }
+
+  template <typename Q>
+  void waf_groups(Q **fld[], int n, Q **outs, Q **wzs, double dta) {
+    for (int b = 0; b < n; b += MAXF) {
+      int m = n - b < MAXF ? n - b : MAXF;
+      switch (m) {
+        case 1: waf_group<Q, 1>(fld + b, outs, wzs, dta); break;
+        case 2: waf_group<Q, 2>(fld + b, outs, wzs, dta); break;
+        case 3: waf_group<Q, 3>(fld + b, outs, wzs, dta); break;
+        case 4: waf_group<Q, 4>(fld + b, outs, wzs, dta); break;
+        default: waf_group<Q, 5>(fld + b, outs, wzs, dta); break;
+      }
+    }
+  }
+
   void advection(double dta) {
     halo(D.u, g.nk, 2);
     halo(D.v, g.nk, 3);
     launch(KK_ADV_IN, fbytes(9, 0), k_adv_in<T>, col_grid(), blk(), D);
-    wafone<T>(D.tetav, D.spare, dta);
-    wafone<double>(D.pai, D.smd, dta);
-    wafone<T>(D.ux, D.spare, dta);
-    wafone<T>(D.vx, D.spare, dta);
-    wafone<T>(D.wx, D.spare, dta);
-    wafone<T>(D.qv, D.spare, dta);
-    T *Dev<T>::*qm[5] = {&Dev<T>::qv, &Dev<T>::qc, &Dev<T>::qi, &Dev<T>::qr, &Dev<T>::qs};
-    for (int n = prm.iqfrst; n <= dims.nqx; n++) wafone<T>(D.*qm[n - 1], D.spare, dta);
+    if (prm.vadvtwice && !unfused_advection) {
+      T **qm[5] = {&D.qv, &D.qc, &D.qi, &D.qr, &D.qs};
+      T **list[10];
+      int n = 0;
+      list[n++] = &D.tetav;
+      if constexpr (sizeof(T) == sizeof(double)) list[n++] = reinterpret_cast<T **>(&D.pai);
+      list[n++] = &D.ux;
+      list[n++] = &D.vx;
+      list[n++] = &D.wx;
+      list[n++] = &D.qv;
+      for (int s = prm.iqfrst; s <= dims.nqx; s++) list[n++] = qm[s - 1];
+      waf_groups<T>(list, n, gout, gwz, dta);
+      if constexpr (sizeof(T) != sizeof(double)) {
+        double **dl[1] = {&D.pai};
+        double *dout[1] = {D.smd}, *dwz[1] = {D.wz};
+        waf_groups<double>(dl, 1, dout, dwz, dta);
+        D.smd = dout[0];
+      }
+      halo(D.ux, g.nk, 2);
+      halo(D.vx, g.nk, 3);
+      launch(KK_ADV_OUT, fbytes(6, 0), k_adv_out<T>, col_grid(), blk(), D);
+      return;
+    }
+    wafone<T>(D.tetav, dta);
+    wafone<double>(D.pai, dta);
+    wafone<T>(D.ux, dta);
+    wafone<T>(D.vx, dta);
+    wafone<T>(D.wx, dta);
+    wafone<T>(D.qv, dta);
+    T *q[5] = {D.qv, D.qc, D.qi, D.qr, D.qs};
+    for (int n = prm.iqfrst; n <= dims.nqx; n++) wafone<T>(q[n - 1], dta);
     halo(D.ux, g.nk, 2);
     halo(D.vx, g.nk, 3);
     launch(KK_ADV_OUT, fbytes(6, 0), k_adv_out<T>, col_grid(), blk(), D);
