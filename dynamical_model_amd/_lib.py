@@ -95,6 +95,15 @@ EXPORTS = {
     "mlc_case_params": (C.c_int, [C.POINTER(Case), C.POINTER(Params)]),
     "mlc_case_build": (C.c_int, [C.POINTER(Case), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "mlc_case_load": (C.c_int, [C.c_void_p, C.POINTER(Case)]),
+    "mlc_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_int, C.POINTER(C.c_void_p)]),
+    "mlc_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "mlc_mlp_weights": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
+    "mlc_mlp_gradients": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "mlc_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "mlc_mlp_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
+    "mlc_mlp_adam": (C.c_int, [C.c_void_p, C.c_double]),
+    "mlc_gemm_bf16_host": (C.c_int, [C.c_void_p, C.c_long, C.c_long, C.c_long, C.c_void_p, C.c_long, C.c_long,
+                                     C.c_long, C.c_int, C.c_int, C.c_int, C.c_void_p]),
     "mlc_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "mlc_profile_kinds": (C.c_int, []),
     "mlc_profile_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double),

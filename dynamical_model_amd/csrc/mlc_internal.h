@@ -7,3 +7,34 @@ int mlc_fail(int code, const char *fmt, ...);
 // Host copies of reference functions used during set-up.
 double gzita_host(double zita, double ztop, double a0);  // mod_zita.F90:110-120
 double pfwsat_host(double t, double p);                   // nompi/mod_moloch.F90:119-181
+
+// ------------------------------------------------------------ the MLP engine
+#include <hip/hip_runtime.h>
+#include <vector>
+
+struct mlc_mlp {
+  int nin, nhid, nout;
+  hipStream_t stream = nullptr;
+  // fp32 master weights (row-major [in][out]) and biases, bf16 copies
+  float *W[3] = {}, *bvec[3] = {};
+  __bf16 *Wb[3] = {};
+  float *gW[3] = {}, *gb[3] = {};
+  float *mW[3] = {}, *vW[3] = {}, *mb[3] = {}, *vb[3] = {};
+  long step = 0;
+  // activations of a batch of up to cap rows
+  int cap = 0;
+  __bf16 *H1 = nullptr, *H2 = nullptr, *dH1 = nullptr, *dH2 = nullptr, *dYb = nullptr;
+  float *Y = nullptr, *dYf = nullptr, *dH1f = nullptr, *dH2f = nullptr;
+  double *partial = nullptr;
+  std::vector<void *> allocs;
+  int in_dim(int l) const { return l == 0 ? nin : nhid; }
+  int out_dim(int l) const { return l == 2 ? nout : nhid; }
+};
+
+// Forward of rows X [rows x nin] (bf16, row stride ldx) into Yout (fp32,
+// row stride ldy; the engine's own buffer when Yout is null).
+int mlp_forward_dev(mlc_mlp *p, const __bf16 *X, long ldx, int rows, float *Yout, long ldy);
+// Loss (mean square error) and gradients of one batch with fp32 targets T.
+int mlp_backward_dev(mlc_mlp *p, const __bf16 *X, long ldx, const float *T, int rows, double *loss);
+// One Adam step with the gradients of the last backward.
+int mlp_adam_dev(mlc_mlp *p, double lr);

@@ -1,0 +1,522 @@
+// mlp.hip -- column-stencil MLP of the learned acoustic corrector on MFMA.
+//
+// The reference has no learned component; this network is new.  It maps a
+// 3 x 3 stencil of columns (6 fields x kz levels each) to corrections of u,
+// v, w and the Exner function (4 x kz) that move a reduced acoustic loop
+// toward the full one.  Layers: nin -> nhid -> nhid -> nout, ReLU between,
+// BF16 operands on the matrix cores with FP32 accumulation, FP32 master
+// weights and Adam.
+//
+// GEMM: C[M x N] = A[M x K] . B[K x N] with arbitrary element strides for A
+// and B (so transposed operands need no copy), on
+// v_mfma_f32_32x32x16_bf16.  A block computes a 128 x 128 tile with four
+// waves of 64 x 64; K advances 32 at a time through LDS.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "mlc_internal.h"
+#include "moloch_hip.h"
+
+namespace mlc {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+
+__device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
+__device__ __forceinline__ float from_bf16(__bf16 x) { return (float)x; }
+
+constexpr int GBM = 128, GBN = 128, GBK = 32;
+constexpr int GLDK = GBK + 8;  // padded LDS row (bf16 elements)
+
+struct GemmArgs {
+  const __bf16 *A;
+  long sam, sak;  // element (m,k) at A[m*sam + k*sak]
+  const __bf16 *B;
+  long sbk, sbn;  // element (k,n) at B[k*sbk + n*sbn]
+  int M, N, K;
+  const float *bias;  // per n, or null
+  int relu;           // apply max(0, .) after the bias
+  float *Cf;          // fp32 output (ldc), or null
+  __bf16 *Cb;         // bf16 output (ldc), or null
+  long ldc;
+  int accumulate;     // Cf += result instead of Cf = result
+};
+
+// Threads stage a 128 x 32 tile of A as LDS [m][k] and a 32 x 128 tile of B
+// as LDS [n][k], so every MFMA lane reads 8 consecutive k with one 16-byte
+// load.  Global reads follow whichever index of the operand is contiguous.
+__global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs a) {
+  __shared__ __bf16 sA[2][GBM][GLDK];
+  __shared__ __bf16 sB[2][GBN][GLDK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int m0 = blockIdx.y * GBM, n0 = blockIdx.x * GBN;
+  const bool a_kfast = a.sak == 1, b_kfast = a.sbk == 1;
+
+  auto load_tiles = [&](int kt, int buf) {
+    const int k0 = kt * GBK;
+    // 4096 elements of each operand, 16 per thread
+    for (int e = 0; e < 16; e++) {
+      int idx = tid + 256 * e;
+      int mm, kk;
+      if (a_kfast) {
+        kk = idx % GBK;
+        mm = idx / GBK;
+      } else {
+        mm = idx % GBM;
+        kk = idx / GBM;
+      }
+      int gm = m0 + mm, gk = k0 + kk;
+      __bf16 v = (gm < a.M && gk < a.K) ? a.A[(long)gm * a.sam + (long)gk * a.sak] : to_bf16(0.f);
+      sA[buf][mm][kk] = v;
+      int nn;
+      if (b_kfast) {
+        kk = idx % GBK;
+        nn = idx / GBK;
+      } else {
+        nn = idx % GBN;
+        kk = idx / GBN;
+      }
+      int gn = n0 + nn;
+      gk = k0 + kk;
+      __bf16 w = (gn < a.N && gk < a.K) ? a.B[(long)gk * a.sbk + (long)gn * a.sbn] : to_bf16(0.f);
+      sB[buf][nn][kk] = w;
+    }
+  };
+
+  f32x16 acc[2][2];
+  for (int x = 0; x < 2; x++)
+    for (int y = 0; y < 2; y++)
+      for (int r = 0; r < 16; r++) acc[x][y][r] = 0.f;
+
+  const int nkt = (a.K + GBK - 1) / GBK;
+  load_tiles(0, 0);
+  __syncthreads();
+  for (int kt = 0; kt < nkt; kt++) {
+    const int buf = kt & 1;
+    if (kt + 1 < nkt) load_tiles(kt + 1, buf ^ 1);
+    // two MFMA K-steps of 16 per LDS tile
+    for (int ks = 0; ks < GBK; ks += 16) {
+      const int kq = ks + 8 * (lane >> 5);
+      bf16x8 av[2], bv[2];
+      for (int x = 0; x < 2; x++) av[x] = *reinterpret_cast<const bf16x8 *>(&sA[buf][wm + 32 * x + (lane & 31)][kq]);
+      for (int y = 0; y < 2; y++) bv[y] = *reinterpret_cast<const bf16x8 *>(&sB[buf][wn + 32 * y + (lane & 31)][kq]);
+      for (int x = 0; x < 2; x++)
+        for (int y = 0; y < 2; y++) acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[x], bv[y], acc[x][y], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+  // D[m][n]: n = lane % 32, m = 8*(r/4) + 4*(lane/32) + r%4
+  for (int x = 0; x < 2; x++)
+    for (int y = 0; y < 2; y++)
+      for (int r = 0; r < 16; r++) {
+        int m = m0 + wm + 32 * x + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        int n = n0 + wn + 32 * y + (lane & 31);
+        if (m >= a.M || n >= a.N) continue;
+        float v = acc[x][y][r];
+        if (a.bias) v += a.bias[n];
+        if (a.relu) v = v > 0.f ? v : 0.f;
+        long o = (long)m * a.ldc + n;
+        if (a.Cf) a.Cf[o] = a.accumulate ? a.Cf[o] + v : v;
+        if (a.Cb) a.Cb[o] = to_bf16(v);
+      }
+}
+
+static int gemm(hipStream_t st, const GemmArgs &a) {
+  if (a.M <= 0 || a.N <= 0 || a.K <= 0) return MLC_OK;
+  dim3 grid((a.N + GBN - 1) / GBN, (a.M + GBM - 1) / GBM);
+  hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return mlc_fail(MLC_EHIP, "gemm launch: %s", hipGetErrorString(e));
+  return MLC_OK;
+}
+
+// ------------------------------------------------------ elementwise kernels
+__global__ void k_f32_to_bf16(const float *x, __bf16 *y, long n) {
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t < n) y[t] = to_bf16(x[t]);
+}
+
+// loss = sum (y - t)^2 / (rows * cols); dY = 2 (y - t) / (rows * cols) as bf16;
+// per-block partial sums of the squared error.
+__global__ void __launch_bounds__(256) k_mse(const float *y, const float *t, __bf16 *dy, float *dyf, long n,
+                                             float scale, double *partial) {
+  __shared__ double sh[256];
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  double e2 = 0.0;
+  if (idx < n) {
+    float d = y[idx] - t[idx];
+    e2 = (double)d * d;
+    float g = 2.f * d * scale;
+    dy[idx] = to_bf16(g);
+    if (dyf) dyf[idx] = g;
+  }
+  sh[threadIdx.x] = e2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// dH = dH_in * (H > 0), in place on a bf16 gradient, H the post-ReLU bf16
+__global__ void k_relu_bwd(__bf16 *dh, const __bf16 *h, float *dhf, long n) {
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  float g = from_bf16(h[t]) > 0.f ? (dhf ? dhf[t] : from_bf16(dh[t])) : 0.f;
+  dh[t] = to_bf16(g);
+  if (dhf) dhf[t] = g;
+}
+
+// db[n] = sum over rows of dY[row][n] (fp32 dY)
+__global__ void __launch_bounds__(256) k_colsum(const float *dy, float *db, int rows, int cols) {
+  int n = blockIdx.x;
+  double s = 0.0;
+  for (int r = threadIdx.x; r < rows; r += 256) s += dy[(long)r * cols + n];
+  __shared__ double sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[n] = (float)sh[0];
+}
+
+// Adam on fp32 master weights, refreshing the bf16 copy
+__global__ void k_adam(float *w, __bf16 *wb, const float *g, float *m, float *v, long n, float lr, float b1,
+                       float b2, float eps, float c1, float c2) {
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  float gt = g[t];
+  float mt = b1 * m[t] + (1.f - b1) * gt;
+  float vt = b2 * v[t] + (1.f - b2) * gt * gt;
+  m[t] = mt;
+  v[t] = vt;
+  float wt = w[t] - lr * (mt / c1) / (sqrtf(vt / c2) + eps);
+  w[t] = wt;
+  if (wb) wb[t] = to_bf16(wt);
+}
+
+}  // namespace mlc
+
+using namespace mlc;
+
+// The MLP object (struct mlc_mlp) is declared in mlc_internal.h.
+
+static int mlp_alloc(mlc_mlp *p, void **ptr, size_t bytes) {
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) return mlc_fail(MLC_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  p->allocs.push_back(*ptr);
+  e = hipMemsetAsync(*ptr, 0, bytes, p->stream);
+  if (e != hipSuccess) return mlc_fail(MLC_EHIP, "hipMemsetAsync: %s", hipGetErrorString(e));
+  return MLC_OK;
+}
+
+static int mlp_reserve(mlc_mlp *p, int rows) {
+  if (rows <= p->cap) return MLC_OK;
+  for (void *q : {(void *)p->H1, (void *)p->H2, (void *)p->dH1, (void *)p->dH2, (void *)p->dYb, (void *)p->Y,
+                  (void *)p->dYf, (void *)p->dH1f, (void *)p->dH2f, (void *)p->partial}) {
+    if (!q) continue;
+    for (size_t i = 0; i < p->allocs.size(); i++)
+      if (p->allocs[i] == q) {
+        (void)hipFree(q);
+        p->allocs.erase(p->allocs.begin() + i);
+        break;
+      }
+  }
+  long r = rows;
+  int rc;
+  if ((rc = mlp_alloc(p, (void **)&p->H1, r * p->nhid * sizeof(__bf16)))) return rc;
+  if ((rc = mlp_alloc(p, (void **)&p->H2, r * p->nhid * sizeof(__bf16)))) return rc;
+  if ((rc = mlp_alloc(p, (void **)&p->dH1, r * p->nhid * sizeof(__bf16)))) return rc;
+  if ((rc = mlp_alloc(p, (void **)&p->dH2, r * p->nhid * sizeof(__bf16)))) return rc;
+  if ((rc = mlp_alloc(p, (void **)&p->dYb, r * p->nout * sizeof(__bf16)))) return rc;
+  if ((rc = mlp_alloc(p, (void **)&p->Y, r * p->nout * sizeof(float)))) return rc;
+  if ((rc = mlp_alloc(p, (void **)&p->dYf, r * p->nout * sizeof(float)))) return rc;
+  if ((rc = mlp_alloc(p, (void **)&p->dH1f, r * p->nhid * sizeof(float)))) return rc;
+  if ((rc = mlp_alloc(p, (void **)&p->dH2f, r * p->nhid * sizeof(float)))) return rc;
+  if ((rc = mlp_alloc(p, (void **)&p->partial, ((r * p->nout + 255) / 256 + 1) * sizeof(double)))) return rc;
+  p->cap = rows;
+  return MLC_OK;
+}
+
+#define MLP_TRY(x)                                                                           \
+  do {                                                                                       \
+    hipError_t e_ = (x);                                                                     \
+    if (e_ != hipSuccess) return mlc_fail(MLC_EHIP, "%s: %s", #x, hipGetErrorString(e_));    \
+  } while (0)
+
+// Forward of rows X[rows x nin] (bf16, row stride ldx): H1, H2, Y (fp32)
+int mlp_forward_dev(mlc_mlp *p, const __bf16 *X, long ldx, int rows, float *Yout, long ldy) {
+  int rc = mlp_reserve(p, rows);
+  if (rc) return rc;
+  GemmArgs a{};
+  a.A = X, a.sam = ldx, a.sak = 1, a.B = p->Wb[0], a.sbk = p->nhid, a.sbn = 1;
+  a.M = rows, a.N = p->nhid, a.K = p->nin, a.bias = p->bvec[0], a.relu = 1, a.Cb = p->H1, a.ldc = p->nhid;
+  if ((rc = gemm(p->stream, a))) return rc;
+  a = GemmArgs{};
+  a.A = p->H1, a.sam = p->nhid, a.sak = 1, a.B = p->Wb[1], a.sbk = p->nhid, a.sbn = 1;
+  a.M = rows, a.N = p->nhid, a.K = p->nhid, a.bias = p->bvec[1], a.relu = 1, a.Cb = p->H2, a.ldc = p->nhid;
+  if ((rc = gemm(p->stream, a))) return rc;
+  a = GemmArgs{};
+  a.A = p->H2, a.sam = p->nhid, a.sak = 1, a.B = p->Wb[2], a.sbk = p->nout, a.sbn = 1;
+  a.M = rows, a.N = p->nout, a.K = p->nhid, a.bias = p->bvec[2], a.relu = 0, a.Cf = Yout ? Yout : p->Y;
+  a.ldc = Yout ? ldy : p->nout;
+  return gemm(p->stream, a);
+}
+
+// Loss and gradients of one batch: X (bf16) and targets T (fp32, rows x nout).
+int mlp_backward_dev(mlc_mlp *p, const __bf16 *X, long ldx, const float *T, int rows, double *loss) {
+  int rc = mlp_forward_dev(p, X, ldx, rows, nullptr, 0);
+  if (rc) return rc;
+  const long ny = (long)rows * p->nout, nh = (long)rows * p->nhid;
+  const int nblk = (int)((ny + 255) / 256);
+  hipLaunchKernelGGL(k_mse, dim3(nblk), dim3(256), 0, p->stream, p->Y, T, p->dYb, p->dYf, ny,
+                     1.0f / (float)ny, p->partial);
+  // layer 3: gW3 = H2^T dY, gb3 = colsum dY, dH2 = dY W3^T
+  GemmArgs a{};
+  a.A = p->H2, a.sam = 1, a.sak = p->nhid, a.B = p->dYb, a.sbk = p->nout, a.sbn = 1;
+  a.M = p->nhid, a.N = p->nout, a.K = rows, a.Cf = p->gW[2], a.ldc = p->nout;
+  if ((rc = gemm(p->stream, a))) return rc;
+  hipLaunchKernelGGL(k_colsum, dim3(p->nout), dim3(256), 0, p->stream, p->dYf, p->gb[2], rows, p->nout);
+  a = GemmArgs{};
+  a.A = p->dYb, a.sam = p->nout, a.sak = 1, a.B = p->Wb[2], a.sbk = 1, a.sbn = p->nout;
+  a.M = rows, a.N = p->nhid, a.K = p->nout, a.Cf = p->dH2f, a.ldc = p->nhid;
+  if ((rc = gemm(p->stream, a))) return rc;
+  hipLaunchKernelGGL(k_relu_bwd, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, p->stream, p->dH2, p->H2,
+                     p->dH2f, nh);
+  // layer 2
+  a = GemmArgs{};
+  a.A = p->H1, a.sam = 1, a.sak = p->nhid, a.B = p->dH2, a.sbk = p->nhid, a.sbn = 1;
+  a.M = p->nhid, a.N = p->nhid, a.K = rows, a.Cf = p->gW[1], a.ldc = p->nhid;
+  if ((rc = gemm(p->stream, a))) return rc;
+  hipLaunchKernelGGL(k_colsum, dim3(p->nhid), dim3(256), 0, p->stream, p->dH2f, p->gb[1], rows, p->nhid);
+  a = GemmArgs{};
+  a.A = p->dH2, a.sam = p->nhid, a.sak = 1, a.B = p->Wb[1], a.sbk = 1, a.sbn = p->nhid;
+  a.M = rows, a.N = p->nhid, a.K = p->nhid, a.Cf = p->dH1f, a.ldc = p->nhid;
+  if ((rc = gemm(p->stream, a))) return rc;
+  hipLaunchKernelGGL(k_relu_bwd, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, p->stream, p->dH1, p->H1,
+                     p->dH1f, nh);
+  // layer 1
+  a = GemmArgs{};
+  a.A = X, a.sam = 1, a.sak = ldx, a.B = p->dH1, a.sbk = p->nhid, a.sbn = 1;
+  a.M = p->nin, a.N = p->nhid, a.K = rows, a.Cf = p->gW[0], a.ldc = p->nhid;
+  if ((rc = gemm(p->stream, a))) return rc;
+  hipLaunchKernelGGL(k_colsum, dim3(p->nhid), dim3(256), 0, p->stream, p->dH1f, p->gb[0], rows, p->nhid);
+  if (loss) {
+    std::vector<double> part(nblk);
+    MLP_TRY(hipMemcpyAsync(part.data(), p->partial, nblk * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    MLP_TRY(hipStreamSynchronize(p->stream));
+    double s = 0.0;
+    for (double x : part) s += x;
+    *loss = s / (double)ny;
+  }
+  return MLC_OK;
+}
+
+int mlp_adam_dev(mlc_mlp *p, double lr) {
+  p->step++;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float c1 = 1.f - powf(b1, (float)p->step), c2 = 1.f - powf(b2, (float)p->step);
+  for (int l = 0; l < 3; l++) {
+    long nw = (long)p->in_dim(l) * p->out_dim(l), nb = p->out_dim(l);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, p->stream, p->W[l], p->Wb[l],
+                       p->gW[l], p->mW[l], p->vW[l], nw, (float)lr, b1, b2, eps, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, p->stream, p->bvec[l],
+                       (__bf16 *)nullptr, p->gb[l], p->mb[l], p->vb[l], nb, (float)lr, b1, b2, eps, c1, c2);
+  }
+  MLP_TRY(hipGetLastError());
+  return MLC_OK;
+}
+
+static int mlp_sync_bf16(mlc_mlp *p) {
+  for (int l = 0; l < 3; l++) {
+    long nw = (long)p->in_dim(l) * p->out_dim(l);
+    hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, p->stream, p->W[l],
+                       p->Wb[l], nw);
+  }
+  MLP_TRY(hipGetLastError());
+  return MLC_OK;
+}
+
+// ================================================================== C ABI
+extern "C" int mlc_mlp_create(int nin, int nhid, int nout, unsigned long long seed, int device, mlc_mlp **out) {
+  if (!out || nin <= 0 || nhid <= 0 || nout <= 0) return mlc_fail(MLC_EINVAL, "mlc_mlp_create: bad arguments");
+  *out = nullptr;
+  MLP_TRY(hipSetDevice(device));
+  mlc_mlp *p = new mlc_mlp();
+  p->nin = nin, p->nhid = nhid, p->nout = nout;
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete p;
+    return mlc_fail(MLC_EHIP, "mlc_mlp_create: stream");
+  }
+  int rc = MLC_OK;
+  // He-uniform initial weights from splitmix64, zero biases
+  uint64_t s = seed;
+  auto rnd = [&]() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  for (int l = 0; l < 3 && !rc; l++) {
+    long nw = (long)p->in_dim(l) * p->out_dim(l), nb = p->out_dim(l);
+    if ((rc = mlp_alloc(p, (void **)&p->W[l], nw * 4)) || (rc = mlp_alloc(p, (void **)&p->Wb[l], nw * 2)) ||
+        (rc = mlp_alloc(p, (void **)&p->gW[l], nw * 4)) || (rc = mlp_alloc(p, (void **)&p->mW[l], nw * 4)) ||
+        (rc = mlp_alloc(p, (void **)&p->vW[l], nw * 4)) || (rc = mlp_alloc(p, (void **)&p->bvec[l], nb * 4)) ||
+        (rc = mlp_alloc(p, (void **)&p->gb[l], nb * 4)) || (rc = mlp_alloc(p, (void **)&p->mb[l], nb * 4)) ||
+        (rc = mlp_alloc(p, (void **)&p->vb[l], nb * 4)))
+      break;
+    std::vector<float> w(nw);
+    double lim = std::sqrt(6.0 / (double)p->in_dim(l)) * (l == 2 ? 0.1 : 1.0);
+    for (long i = 0; i < nw; i++) w[i] = (float)((2.0 * rnd() - 1.0) * lim);
+    if (hipMemcpy(p->W[l], w.data(), nw * 4, hipMemcpyHostToDevice) != hipSuccess)
+      rc = mlc_fail(MLC_EHIP, "mlc_mlp_create: upload");
+  }
+  if (!rc) rc = mlp_sync_bf16(p);
+  if (!rc && hipStreamSynchronize(p->stream) != hipSuccess) rc = mlc_fail(MLC_EHIP, "mlc_mlp_create: sync");
+  if (rc) {
+    for (void *q : p->allocs) (void)hipFree(q);
+    (void)hipStreamDestroy(p->stream);
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return MLC_OK;
+}
+
+extern "C" int mlc_mlp_destroy(mlc_mlp *p) {
+  if (!p) return MLC_OK;
+  (void)hipStreamSynchronize(p->stream);
+  for (void *q : p->allocs) (void)hipFree(q);
+  (void)hipStreamDestroy(p->stream);
+  delete p;
+  return MLC_OK;
+}
+
+// Copy fp32 weights and biases of layer l (0..2) in or out: w is [in][out].
+extern "C" int mlc_mlp_weights(mlc_mlp *p, int layer, float *w, float *b, int set) {
+  if (!p || layer < 0 || layer > 2) return mlc_fail(MLC_EINVAL, "mlc_mlp_weights: bad arguments");
+  long nw = (long)p->in_dim(layer) * p->out_dim(layer), nb = p->out_dim(layer);
+  hipMemcpyKind kd = set ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+  if (w) MLP_TRY(hipMemcpyAsync(set ? (void *)p->W[layer] : (void *)w, set ? (void *)w : (void *)p->W[layer], nw * 4,
+                                kd, p->stream));
+  if (b) MLP_TRY(hipMemcpyAsync(set ? (void *)p->bvec[layer] : (void *)b, set ? (void *)b : (void *)p->bvec[layer],
+                                nb * 4, kd, p->stream));
+  if (set) {
+    int rc = mlp_sync_bf16(p);
+    if (rc) return rc;
+  }
+  MLP_TRY(hipStreamSynchronize(p->stream));
+  return MLC_OK;
+}
+
+// Gradients of layer l from the last mlc_mlp_grad call.
+extern "C" int mlc_mlp_gradients(mlc_mlp *p, int layer, float *gw, float *gb) {
+  if (!p || layer < 0 || layer > 2) return mlc_fail(MLC_EINVAL, "mlc_mlp_gradients: bad arguments");
+  long nw = (long)p->in_dim(layer) * p->out_dim(layer), nb = p->out_dim(layer);
+  if (gw) MLP_TRY(hipMemcpyAsync(gw, p->gW[layer], nw * 4, hipMemcpyDeviceToHost, p->stream));
+  if (gb) MLP_TRY(hipMemcpyAsync(gb, p->gb[layer], nb * 4, hipMemcpyDeviceToHost, p->stream));
+  MLP_TRY(hipStreamSynchronize(p->stream));
+  return MLC_OK;
+}
+
+// Host-array convenience entry points (tests): X [rows x nin] fp32 (rounded
+// to bf16 on upload), Y [rows x nout] fp32, T [rows x nout] fp32.
+static int upload_bf16(mlc_mlp *p, const float *x, long n, __bf16 **dev) {
+  float *tmp;
+  int rc = mlp_alloc(p, (void **)&tmp, n * 4);
+  if (rc) return rc;
+  rc = mlp_alloc(p, (void **)dev, n * 2);
+  if (rc) return rc;
+  MLP_TRY(hipMemcpyAsync(tmp, x, n * 4, hipMemcpyHostToDevice, p->stream));
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, tmp, *dev, n);
+  MLP_TRY(hipStreamSynchronize(p->stream));
+  for (size_t i = 0; i < p->allocs.size(); i++)
+    if (p->allocs[i] == tmp) {
+      (void)hipFree(tmp);
+      p->allocs.erase(p->allocs.begin() + i);
+      break;
+    }
+  return MLC_OK;
+}
+static void release(mlc_mlp *p, void *q) {
+  for (size_t i = 0; i < p->allocs.size(); i++)
+    if (p->allocs[i] == q) {
+      (void)hipFree(q);
+      p->allocs.erase(p->allocs.begin() + i);
+      return;
+    }
+}
+
+extern "C" int mlc_mlp_forward(mlc_mlp *p, const float *X, int rows, float *Y) {
+  if (!p || !X || !Y || rows <= 0) return mlc_fail(MLC_EINVAL, "mlc_mlp_forward: bad arguments");
+  __bf16 *xd = nullptr;
+  int rc = upload_bf16(p, X, (long)rows * p->nin, &xd);
+  if (rc) return rc;
+  rc = mlp_forward_dev(p, xd, p->nin, rows, nullptr, 0);
+  if (!rc) {
+    if (hipMemcpyAsync(Y, p->Y, (long)rows * p->nout * 4, hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess)
+      rc = mlc_fail(MLC_EHIP, "mlc_mlp_forward: download");
+  }
+  release(p, xd);
+  return rc;
+}
+
+extern "C" int mlc_mlp_grad(mlc_mlp *p, const float *X, const float *T, int rows, double *loss) {
+  if (!p || !X || !T || rows <= 0) return mlc_fail(MLC_EINVAL, "mlc_mlp_grad: bad arguments");
+  __bf16 *xd = nullptr;
+  float *td = nullptr;
+  int rc = upload_bf16(p, X, (long)rows * p->nin, &xd);
+  if (rc) return rc;
+  if ((rc = mlp_alloc(p, (void **)&td, (long)rows * p->nout * 4))) return rc;
+  MLP_TRY(hipMemcpyAsync(td, T, (long)rows * p->nout * 4, hipMemcpyHostToDevice, p->stream));
+  rc = mlp_backward_dev(p, xd, p->nin, td, rows, loss);
+  if (!rc && hipStreamSynchronize(p->stream) != hipSuccess) rc = mlc_fail(MLC_EHIP, "mlc_mlp_grad: sync");
+  release(p, xd);
+  release(p, td);
+  return rc;
+}
+
+extern "C" int mlc_mlp_adam(mlc_mlp *p, double lr) {
+  if (!p) return mlc_fail(MLC_EINVAL, "mlc_mlp_adam: NULL");
+  int rc = mlp_adam_dev(p, lr);
+  if (rc) return rc;
+  MLP_TRY(hipStreamSynchronize(p->stream));
+  return MLC_OK;
+}
+
+// C = A . B for host fp32 matrices rounded to bf16 (tests of the GEMM):
+// A [M x K] with strides (sam, sak), B [K x N] with (sbk, sbn), C [M x N].
+extern "C" int mlc_gemm_bf16_host(const float *A, long sam, long sak, long asize, const float *B, long sbk, long sbn,
+                                  long bsize, int M, int N, int K, float *C) {
+  mlc_mlp tmp;
+  tmp.nin = tmp.nhid = tmp.nout = 1;
+  MLP_TRY(hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking));
+  __bf16 *ad = nullptr, *bd = nullptr;
+  float *cd = nullptr;
+  int rc = upload_bf16(&tmp, A, asize, &ad);
+  if (!rc) rc = upload_bf16(&tmp, B, bsize, &bd);
+  if (!rc) rc = mlp_alloc(&tmp, (void **)&cd, (long)M * N * 4);
+  if (!rc) {
+    GemmArgs a{};
+    a.A = ad, a.sam = sam, a.sak = sak, a.B = bd, a.sbk = sbk, a.sbn = sbn, a.M = M, a.N = N, a.K = K;
+    a.Cf = cd, a.ldc = N;
+    rc = gemm(tmp.stream, a);
+  }
+  if (!rc && (hipMemcpyAsync(C, cd, (long)M * N * 4, hipMemcpyDeviceToHost, tmp.stream) != hipSuccess ||
+              hipStreamSynchronize(tmp.stream) != hipSuccess))
+    rc = mlc_fail(MLC_EHIP, "mlc_gemm_bf16_host: download");
+  for (void *q : tmp.allocs) (void)hipFree(q);
+  (void)hipStreamDestroy(tmp.stream);
+  return rc;
+}
