@@ -80,6 +80,24 @@ class Case(C.Structure):
     ]
 
 
+class LacConfig(C.Structure):
+    _fields_ = [
+        ("nhid", C.c_int), ("samples_per_phase", C.c_int), ("capacity", C.c_int),
+        ("chunk_rows", C.c_int), ("check_every", C.c_int), ("check_tol", C.c_double),
+        ("seed", C.c_ulonglong),
+    ]
+
+
+class LacStats(C.Structure):
+    _fields_ = [
+        ("samples", C.c_longlong), ("learned_phases", C.c_longlong),
+        ("plain_phases", C.c_longlong), ("checks", C.c_longlong), ("fallback", C.c_int),
+        ("last_loss", C.c_double), ("ratio", C.c_double * 4), ("rel", C.c_double * 4),
+    ]
+
+
+MLC_LAC_OFF, MLC_LAC_ON, MLC_LAC_REDUCED = 0, 1, 2
+
 EXPORTS = {
     "mlc_default_params": (C.c_int, [C.POINTER(Params)]),
     "mlc_create": (C.c_int, [C.POINTER(Dims), C.POINTER(Params), C.POINTER(C.c_void_p)]),
@@ -104,6 +122,14 @@ EXPORTS = {
     "mlc_mlp_adam": (C.c_int, [C.c_void_p, C.c_double]),
     "mlc_gemm_bf16_host": (C.c_int, [C.c_void_p, C.c_long, C.c_long, C.c_long, C.c_void_p, C.c_long, C.c_long,
                                      C.c_long, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "mlc_lac_default_config": (C.c_int, [C.POINTER(LacConfig)]),
+    "mlc_lac_create": (C.c_int, [C.c_void_p, C.POINTER(LacConfig)]),
+    "mlc_lac_collect": (C.c_int, [C.c_void_p, C.c_int]),
+    "mlc_lac_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double)]),
+    "mlc_lac_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "mlc_lac_get_stats": (C.c_int, [C.c_void_p, C.POINTER(LacStats)]),
+    "mlc_lac_network": (C.c_void_p, [C.c_void_p]),
+    "mlc_lac_dataset": (C.c_int, [C.c_void_p, C.c_long, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "mlc_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "mlc_profile_kinds": (C.c_int, []),
     "mlc_profile_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double),

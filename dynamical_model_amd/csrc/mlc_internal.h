@@ -15,6 +15,7 @@ double pfwsat_host(double t, double p);                   // nompi/mod_moloch.F9
 struct mlc_mlp {
   int nin, nhid, nout;
   hipStream_t stream = nullptr;
+  bool own_stream = true;  // false when the stream belongs to a model handle
   // fp32 master weights (row-major [in][out]) and biases, bf16 copies
   float *W[3] = {}, *bvec[3] = {};
   __bf16 *Wb[3] = {};
@@ -31,6 +32,11 @@ struct mlc_mlp {
   int out_dim(int l) const { return l == 2 ? nout : nhid; }
 };
 
+// A network on device `device`; with stream == nullptr it creates and owns a
+// stream, otherwise it queues its work on `stream`.
+int mlp_new(int nin, int nhid, int nout, unsigned long long seed, int device, hipStream_t stream, mlc_mlp **out);
+// Release the device memory (and an owned stream) of a network.
+void mlp_free(mlc_mlp *p);
 // Forward of rows X [rows x nin] (bf16, row stride ldx) into Yout (fp32,
 // row stride ldy; the engine's own buffer when Yout is null).
 int mlp_forward_dev(mlc_mlp *p, const __bf16 *X, long ldx, int rows, float *Yout, long ldy);

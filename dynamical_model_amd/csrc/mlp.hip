@@ -19,7 +19,7 @@
 #include <vector>
 
 #include "mlc_internal.h"
-#include "moloch_hip.h"
+#include "moloch_lac.h"
 
 namespace mlc {
 
@@ -346,14 +346,16 @@ static int mlp_sync_bf16(mlc_mlp *p) {
   return MLC_OK;
 }
 
-// ================================================================== C ABI
-extern "C" int mlc_mlp_create(int nin, int nhid, int nout, unsigned long long seed, int device, mlc_mlp **out) {
+int mlp_new(int nin, int nhid, int nout, unsigned long long seed, int device, hipStream_t stream, mlc_mlp **out) {
   if (!out || nin <= 0 || nhid <= 0 || nout <= 0) return mlc_fail(MLC_EINVAL, "mlc_mlp_create: bad arguments");
   *out = nullptr;
   MLP_TRY(hipSetDevice(device));
   mlc_mlp *p = new mlc_mlp();
   p->nin = nin, p->nhid = nhid, p->nout = nout;
-  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (stream) {
+    p->stream = stream;
+    p->own_stream = false;
+  } else if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
     delete p;
     return mlc_fail(MLC_EHIP, "mlc_mlp_create: stream");
   }
@@ -378,27 +380,35 @@ extern "C" int mlc_mlp_create(int nin, int nhid, int nout, unsigned long long se
     std::vector<float> w(nw);
     double lim = std::sqrt(6.0 / (double)p->in_dim(l)) * (l == 2 ? 0.1 : 1.0);
     for (long i = 0; i < nw; i++) w[i] = (float)((2.0 * rnd() - 1.0) * lim);
-    if (hipMemcpy(p->W[l], w.data(), nw * 4, hipMemcpyHostToDevice) != hipSuccess)
+    if (hipMemcpyAsync(p->W[l], w.data(), nw * 4, hipMemcpyHostToDevice, p->stream) != hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess)
       rc = mlc_fail(MLC_EHIP, "mlc_mlp_create: upload");
   }
   if (!rc) rc = mlp_sync_bf16(p);
   if (!rc && hipStreamSynchronize(p->stream) != hipSuccess) rc = mlc_fail(MLC_EHIP, "mlc_mlp_create: sync");
   if (rc) {
-    for (void *q : p->allocs) (void)hipFree(q);
-    (void)hipStreamDestroy(p->stream);
-    delete p;
+    mlp_free(p);
     return rc;
   }
   *out = p;
   return MLC_OK;
 }
 
-extern "C" int mlc_mlp_destroy(mlc_mlp *p) {
-  if (!p) return MLC_OK;
+void mlp_free(mlc_mlp *p) {
+  if (!p) return;
   (void)hipStreamSynchronize(p->stream);
   for (void *q : p->allocs) (void)hipFree(q);
-  (void)hipStreamDestroy(p->stream);
+  if (p->own_stream) (void)hipStreamDestroy(p->stream);
   delete p;
+}
+
+// ================================================================== C ABI
+extern "C" int mlc_mlp_create(int nin, int nhid, int nout, unsigned long long seed, int device, mlc_mlp **out) {
+  return mlp_new(nin, nhid, nout, seed, device, nullptr, out);
+}
+
+extern "C" int mlc_mlp_destroy(mlc_mlp *p) {
+  mlp_free(p);
   return MLC_OK;
 }
 

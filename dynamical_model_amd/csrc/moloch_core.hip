@@ -34,7 +34,7 @@
 
 #include "mlc_constants.h"
 #include "mlc_internal.h"
-#include "moloch_hip.h"
+#include "moloch_lac.h"
 
 // ------------------------------------------------------------------ errors
 static thread_local char g_errbuf[512] = "no error";
@@ -1103,6 +1103,239 @@ __global__ void __launch_bounds__(256) k_epilogue(Dev<T> D) {
   D.ps[g.at2(j, i)] = (T)(pk * exp(zdgz / (kRgas * tv)));
 }
 
+// =================================================== learned acoustic corrector
+// A network corrects one long acoustic substep of (nsound-1)*dts toward
+// nsound-1 ordinary substeps; the last ordinary substep then runs as usual.
+// Inputs of a column (j,i), taken after the long substep: the increments it
+// made to u, v, w (levels 2..kz+1) and the Exner function, and theta and qv,
+// over kz levels at the 3 x 3 columns around it.  "fl" = field * kz +
+// level - 1 is padded to nflp = roundup(6 kz, 16) with zero features.
+// Feature o packs 8 consecutive fl at one stencil position p, (dj, di) =
+// (p%3-1, p/3-1): chunk c = o/8 -> pair of fl groups dg = c/18, half
+// h = (c%18)/9, p = c%9.  Outputs: increments of u, v, w (levels 2..kz+1)
+// and Exner, o = f*kz + k-1.  Rows of the whole grid are ordered by 64 x 4
+// tiles of columns: r -> tile r/256 = (tj, ti) in j-fastest order, then
+// j-run (r%256)/64 and j-offset r%64.
+constexpr int LAC_IN_FIELDS = 6, LAC_OUT_FIELDS = 4;
+
+__host__ __device__ __forceinline__ void lac_feature(int o, int &fl, int &p) {
+  const int c = o >> 3, cc = c % 18;
+  p = cc % 9;
+  fl = (c / 18) * 16 + (cc / 9) * 8 + (o & 7);
+}
+
+struct LacRows {
+  int ntj;  // tiles along j
+  __device__ __forceinline__ void colof(long r, const Geo &g, int &j, int &i) const {
+    const long t = r >> 8;
+    const int w = (int)(r & 255);
+    j = (int)(t % ntj) * 64 + (w & 63) + 1;
+    i = (int)(t / ntj) * 4 + (w >> 6) + 1;
+  }
+};
+
+// Phase-start copies of the fields whose increments are inputs.
+template <typename T>
+struct LacSnap {
+  const T *u, *v, *w;
+  const double *pai;
+};
+
+template <typename T>
+__device__ __forceinline__ float lac_value(const Dev<T> &D, const LacSnap<T> &S, int fl, int j, int i) {
+  const Geo &g = D.g;
+  const int nk = g.nk, f = fl / nk, k = fl % nk + 1;
+  if (f >= LAC_IN_FIELDS) return 0.f;
+  j = j < 1 ? j + g.nj : (j > g.nj ? j - g.nj : j);
+  i = i < 1 ? i + g.ni : (i > g.ni ? i - g.ni : i);
+  const long x = g.at(j, i, k);
+  switch (f) {
+    case 0: return (float)(D.u[x] - S.u[x]);
+    case 1: return (float)(D.v[x] - S.v[x]);
+    case 2: return (float)(D.w[x + g.plane] - S.w[x + g.plane]);
+    case 3: return (float)(D.pai[x] - S.pai[x]);
+    case 4: return (float)D.tetav[x];
+    default: return (float)D.qv[x];
+  }
+}
+
+// Features of 64 rows per block into row-major X (row stride nin): rows
+// cols[r] = (i-1)*nj + (j-1) when cols is given (raw fp32 into Xf), else
+// rows row0 .. row0+nrows-1 of the tile order (normalised bf16
+// (x - mean[o]) * rstd[o] into Xb; rows outside the grid get zeros).
+template <typename T>
+__global__ void __launch_bounds__(256) k_lac_gather(Dev<T> D, LacSnap<T> S, LacRows R, const int *cols, long row0,
+                                                    int nrows, int nin, float *Xf, __bf16 *Xb, const float *mean,
+                                                    const float *rstd) {
+  __shared__ float tile[64][65];
+  const Geo &g = D.g;
+  const int r0 = blockIdx.x * 64;
+  const int tr = threadIdx.x & 63, tq = threadIdx.x >> 6;
+  const int row = r0 + tr;
+  int j = 0, i = 0;
+  bool ok = row < nrows;
+  if (ok) {
+    if (cols) {
+      const int c = cols[row];
+      j = c % g.nj + 1;
+      i = c / g.nj + 1;
+    } else {
+      R.colof(row0 + row, g, j, i);
+      ok = j <= g.nj && i <= g.ni;
+    }
+  }
+  for (int o0 = 0; o0 < nin; o0 += 64) {
+    for (int q = tq; q < 64; q += 4) {
+      const int o = o0 + q;
+      float val = 0.f;
+      if (ok && o < nin) {
+        int fl, p;
+        lac_feature(o, fl, p);
+        val = lac_value(D, S, fl, j + p % 3 - 1, i + p / 3 - 1);
+        if (Xb) val = (val - mean[o]) * rstd[o];
+      }
+      tile[tr][q] = val;
+    }
+    __syncthreads();
+    for (int q = tq; q < 64; q += 4) {
+      const int rr = r0 + q, o = o0 + tr;
+      if (rr < nrows && o < nin) {
+        const float v = tile[q][tr];
+        if (Xb)
+          Xb[(long)rr * nin + o] = (__bf16)v;
+        else
+          Xf[(long)rr * nin + o] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Training targets at columns cols[r]: ordinary-substep result (the state)
+// minus the long-substep result (fu, fv, fw, fpai).
+template <typename T>
+__global__ void __launch_bounds__(256) k_lac_target(Dev<T> D, const int *cols, int nrows, const T *fu, const T *fv,
+                                                    const T *fw, const double *fpai, float *Tt) {
+  const Geo &g = D.g;
+  const int nk = g.nk, nout = LAC_OUT_FIELDS * nk;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)nrows * nout) return;
+  const int r = (int)(t / nout), o = (int)(t % nout);
+  const int k = o % nk + 1, f = o / nk;
+  const int c = cols[r];
+  const long x = g.at(c % g.nj + 1, c / g.nj + 1, k);
+  double d;
+  switch (f) {
+    case 0: d = (double)D.u[x] - (double)fu[x]; break;
+    case 1: d = (double)D.v[x] - (double)fv[x]; break;
+    case 2: d = (double)D.w[x + g.plane] - (double)fw[x + g.plane]; break;
+    default: d = D.pai[x] - fpai[x]; break;
+  }
+  Tt[t] = (float)d;
+}
+
+// Add the de-normalised network outputs Y (tile-ordered rows, row stride
+// nout) to u, v, w and the Exner function; 64 rows (one j-run) per block,
+// transposed through LDS so both the Y reads and the field writes coalesce.
+template <typename T>
+__global__ void __launch_bounds__(256) k_lac_apply(Dev<T> D, LacRows R, const float *Y, const float *omean,
+                                                   const float *ostd) {
+  __shared__ float tile[64][65];
+  const Geo &g = D.g;
+  const int nk = g.nk, nout = LAC_OUT_FIELDS * nk;
+  const long r0 = (long)blockIdx.x * 64;
+  const int tr = threadIdx.x & 63, tq = threadIdx.x >> 6;
+  int j, i;
+  R.colof(r0 + tr, g, j, i);
+  const bool ok = j <= g.nj && i <= g.ni;
+  for (int o0 = 0; o0 < nout; o0 += 64) {
+    for (int q = tq; q < 64; q += 4) {
+      const int o = o0 + tr;
+      tile[q][tr] = o < nout ? Y[(r0 + q) * nout + o] * ostd[o] + omean[o] : 0.f;
+    }
+    __syncthreads();
+    if (ok)
+      for (int q = tq; q < 64; q += 4) {
+        const int o = o0 + q;
+        if (o >= nout) break;
+        const float d = tile[tr][q];
+        const int f = o / nk, k = o % nk + 1;
+        const long x = g.at(j, i, k);
+        switch (f) {
+          case 0: D.u[x] = D.u[x] + (T)d; break;
+          case 1: D.v[x] = D.v[x] + (T)d; break;
+          case 2: D.w[x + g.plane] = D.w[x + g.plane] + (T)d; break;
+          default: D.pai[x] = D.pai[x] + (double)d; break;
+        }
+      }
+    __syncthreads();
+  }
+}
+
+// Per-column mean and standard deviation of a row-major fp32 matrix.
+__global__ void __launch_bounds__(256) k_colstats(const float *X, long rows, int cols, double *mean, double *sd) {
+  const int o = blockIdx.x;
+  double s = 0.0, s2 = 0.0;
+  for (long r = threadIdx.x; r < rows; r += 256) {
+    const double v = X[r * cols + o];
+    s += v;
+    s2 += v * v;
+  }
+  __shared__ double a[256], b[256];
+  a[threadIdx.x] = s;
+  b[threadIdx.x] = s2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) a[threadIdx.x] += a[threadIdx.x + w], b[threadIdx.x] += b[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double m = a[0] / (double)rows, var = b[0] / (double)rows - m * m;
+    mean[o] = m;
+    sd[o] = sqrt(var > 0.0 ? var : 0.0);
+  }
+}
+
+// Training batch: rows idx[r] of the raw set, normalised (bf16 inputs, fp32
+// targets).
+__global__ void __launch_bounds__(256) k_lac_batch(const float *Xraw, const float *Traw, const int *idx, int nin,
+                                                   int nout, const float *imean, const float *irstd,
+                                                   const float *omean, const float *ostd, __bf16 *Xb, float *Tn) {
+  const long r = blockIdx.y;
+  const long s = idx[r];
+  for (int o = blockIdx.x * 256 + threadIdx.x; o < nin; o += gridDim.x * 256)
+    Xb[r * nin + o] = (__bf16)((Xraw[s * nin + o] - imean[o]) * irstd[o]);
+  for (int o = blockIdx.x * 256 + threadIdx.x; o < nout; o += gridDim.x * 256)
+    Tn[r * nout + o] = (Traw[s * nout + o] - omean[o]) / ostd[o];
+}
+
+// Accumulate sum (a - b)^2 and sum b^2 over the interior, levels k0..k1.
+template <typename F>
+__global__ void __launch_bounds__(256) k_diffnorm(const F *a, const F *b, Geo g, int k0, int k1, double *out) {
+  __shared__ double s1[256], s2[256];
+  const int j = blockIdx.x * 64 + threadIdx.x % 64 + 1;
+  const int i = blockIdx.y * 4 + threadIdx.x / 64 + 1;
+  double d2 = 0.0, b2 = 0.0;
+  if (j <= g.nj && i <= g.ni)
+    for (int k = k0; k <= k1; k++) {
+      const long x = g.at(j, i, k);
+      const double d = (double)a[x] - (double)b[x];
+      d2 += d * d;
+      b2 += (double)b[x] * (double)b[x];
+    }
+  s1[threadIdx.x] = d2;
+  s2[threadIdx.x] = b2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s1[threadIdx.x] += s1[threadIdx.x + w], s2[threadIdx.x] += s2[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&out[0], s1[0]);
+    atomicAdd(&out[1], s2[0]);
+  }
+}
+
 // ============================================================= diagnostics
 // :395-419 -- extrema of ps, u, v over the interior.  Extrema skip NaN
 // entries and are NaN only when every entry is NaN.
@@ -1200,11 +1433,13 @@ __global__ void __launch_bounds__(256) k_diag_final(double *partial, long nblk, 
 // Kernel kinds for the per-launch event timing of mlc_profile_*.
 enum KernelKind {
   KK_HALO = 0, KK_PROLOGUE, KK_SOUND_A, KK_SOUND_B, KK_SOUND_B2, KK_SOUND_C, KK_SMOOTH5, KK_FILTER_BRACKET,
-  KK_ADV_IN, KK_WAF_V, KK_WAF_M, KK_WAF_Z, KK_WAF_H, KK_ADV_OUT, KK_EPILOGUE, KK_DIAG, KK_COUNT
+  KK_ADV_IN, KK_WAF_V, KK_WAF_M, KK_WAF_Z, KK_WAF_H, KK_ADV_OUT, KK_EPILOGUE, KK_DIAG,
+  KK_LAC_GATHER, KK_LAC_MLP, KK_LAC_APPLY, KK_LAC_COPY, KK_COUNT
 };
 static const char *kKernelNames[KK_COUNT] = {
     "halo", "prologue", "sound_a", "sound_b", "sound_b2", "sound_c", "smooth5", "filter_bracket",
-    "adv_in", "waf_v", "waf_m", "waf_z", "waf_h", "adv_out", "epilogue", "diag"};
+    "adv_in", "waf_v", "waf_m", "waf_z", "waf_h", "adv_out", "epilogue", "diag",
+    "lac_gather", "lac_mlp", "lac_apply", "lac_copy"};
 
 struct CoreBase {
   virtual ~CoreBase() {}
@@ -1215,6 +1450,13 @@ struct CoreBase {
   virtual int sync() = 0;
   virtual int profile(int enable) = 0;
   virtual int profile_get(int kind, double *total_ms, long long *launches, double *bytes) = 0;
+  virtual int lac_create(const mlc_lac_config &c) = 0;
+  virtual int lac_collect(int nsteps) = 0;
+  virtual int lac_train(int iters, int batch, double lr, double *loss) = 0;
+  virtual int lac_mode(int mode) = 0;
+  virtual int lac_stats(mlc_lac_stats *s) = 0;
+  virtual mlc_mlp *lac_network() = 0;
+  virtual int lac_dataset(long first, int n, float *X, float *T, int *cols) = 0;
 };
 
 static long round_up(long a, long b) { return (a + b - 1) / b * b; }
@@ -1272,6 +1514,12 @@ struct Core : CoreBase {
 
   template <typename Kern, typename... Args>
   void launch(int kind, double bytes, Kern kern, dim3 grid, dim3 block, Args... args) {
+    timed(kind, bytes, [&]() { hipLaunchKernelGGL(kern, grid, block, 0, stream, args...); });
+  }
+
+  // Run fn (work queued on the stream) between two timing events.
+  template <typename Fn>
+  void timed(int kind, double bytes, Fn fn) {
     ProfRec r{kind, bytes, nullptr, nullptr};
     if (prof) {
       if (evnext + 2 > evpool.size()) {
@@ -1290,7 +1538,7 @@ struct Core : CoreBase {
         (void)hipEventRecord(r.a, stream);
       }
     }
-    hipLaunchKernelGGL(kern, grid, block, 0, stream, args...);
+    fn();
     if (prof && r.a) {
       (void)hipEventRecord(r.b, stream);
       pending.push_back(r);
@@ -1328,6 +1576,7 @@ struct Core : CoreBase {
 
   ~Core() override {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (L.net) mlp_free(L.net);
     for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
     for (void *p : allocs) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
@@ -1588,31 +1837,31 @@ struct Core : CoreBase {
     launch(KK_HALO, 0.0, k_halo<F>, dim3((n + 255) / 256, nlev), dim3(256), f, g, mode);
   }
 
-  void sound(double dts) {
+  // One acoustic substep of length dts (the body of the loop at :511-839);
+  // ns is the substep's ordinal, which enters the w solve (:641-657).
+  void substep(double dts, int ns) {
     Par P = par();
-    if (!prm.fulleq) halo(D.tetav, g.nk, 0);
-    for (int ns = 1; ns <= prm.nsound; ns++) {
-      halo(D.u, g.nk, 1);
-      halo(D.v, g.nk, 1);
-      launch(KK_SOUND_A, fbytes(5, 0), k_sound_a<T>, col_grid(), blk(), D, P, dts);
-      if (!prm.divdamp) {
-        // without divdamp the undamped divergence feeds the column directly;
-        // express it as divdamp with zero coefficients is not equivalent, so
-        // refuse at create time instead (see mlc_create).
-      }
-      int split = prm.divfilter ? 1 : 0;
-      launch(KK_SOUND_B, fbytes(18, 2), k_sound_b<T>, col_grid(), blk(), D, P, dts, ns, split);
-      if (split) {
-        // divergence_filter (:423-438) then the Exner update
-        halo(D.zds, g.nk, 0);
-        launch(KK_SMOOTH5, fbytes(2, 0), k_smooth5<T>, pt_grid(), blk(), (const T *)D.zds, D.zdf, g,
-               (const double *)D.xknu, prm.anu2);
-        launch(KK_SOUND_B2, fbytes(1, 2), k_sound_b2<T>, col_grid(), blk(), D, P);
-      }
-      launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T>, col_grid(), blk(), D, P, dts);
-      std::swap(D.u, D.un);
-      std::swap(D.v, D.vn);
+    halo(D.u, g.nk, 1);
+    halo(D.v, g.nk, 1);
+    launch(KK_SOUND_A, fbytes(5, 0), k_sound_a<T>, col_grid(), blk(), D, P, dts);
+    int split = prm.divfilter ? 1 : 0;
+    launch(KK_SOUND_B, fbytes(18, 2), k_sound_b<T>, col_grid(), blk(), D, P, dts, ns, split);
+    if (split) {
+      // divergence_filter (:423-438) then the Exner update
+      halo(D.zds, g.nk, 0);
+      launch(KK_SMOOTH5, fbytes(2, 0), k_smooth5<T>, pt_grid(), blk(), (const T *)D.zds, D.zdf, g,
+             (const double *)D.xknu, prm.anu2);
+      launch(KK_SOUND_B2, fbytes(1, 2), k_sound_b2<T>, col_grid(), blk(), D, P);
     }
+    launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T>, col_grid(), blk(), D, P, dts);
+    std::swap(D.u, D.un);
+    std::swap(D.v, D.vn);
+  }
+
+  // sound (:483-851): nsound substeps of dts.
+  void sound(double dts) {
+    if (!prm.fulleq) halo(D.tetav, g.nk, 0);
+    for (int ns = 1; ns <= prm.nsound; ns++) substep(dts, ns);
   }
 
   // wafone (:929-1311) on one field with the three sweep kernels (the
@@ -1711,18 +1960,378 @@ struct Core : CoreBase {
     launch(KK_ADV_OUT, fbytes(6, 0), k_adv_out<T>, col_grid(), blk(), D);
   }
 
+  // ------------------------------------------- learned acoustic corrector
+  struct LacState {
+    bool exists = false;
+    mlc_lac_config cfg{};
+    mlc_mlp *net = nullptr;
+    int nflp = 0, nin = 0, nout = 0, chunk = 0;
+    LacRows rows{};
+    long rtot = 0;
+    T *su = nullptr, *sv = nullptr, *sw = nullptr, *stet = nullptr;  // phase-start snapshot
+    double *spai = nullptr;
+    T *ru = nullptr, *rv = nullptr, *rw = nullptr;  // long substep without increment
+    double *rpai = nullptr;
+    T *cu = nullptr, *cv = nullptr, *cw = nullptr;  // long substep with increment
+    double *cpai = nullptr;
+    float *Y = nullptr;        // network outputs of every row [rtot x nout]
+    __bf16 *Xc = nullptr;      // features of one inference batch
+    float *Xraw = nullptr, *Traw = nullptr;  // training set (ring)
+    long count = 0, next = 0;
+    int *cols = nullptr, *Craw = nullptr;  // sampled columns; their ring copy
+    std::vector<int> hcols;
+    __bf16 *Xb = nullptr;
+    float *Tn = nullptr;
+    int *bidx = nullptr;
+    long bcap = 0, icap = 0;
+    float *stat = nullptr;  // imean, irstd [nin]; omean, ostd [nout]
+    bool normalised = false;
+    double *norms = nullptr;  // check accumulators
+    int mode = MLC_LAC_OFF;
+    long long learned_steps = 0;
+    mlc_lac_stats st{};
+    uint64_t rng = 0;
+    uint64_t next_u64() {
+      uint64_t z = (rng += 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      return z ^ (z >> 31);
+    }
+  } L;
+
+  int lac_create(const mlc_lac_config &c) override {
+    if (!initialised) return mlc_fail(MLC_ESTATE, "mlc_lac_create before mlc_init");
+    if (L.exists) return mlc_fail(MLC_ESTATE, "mlc_lac_create: the handle already has a corrector");
+    if (prm.nsound < 2) return mlc_fail(MLC_EINVAL, "mlc_lac_create: needs nsound >= 2");
+    if (c.nhid < 16 || c.samples_per_phase < 1 || c.capacity < 1 || c.chunk_rows < 256 || c.chunk_rows % 256 ||
+        c.check_every < 0 || !(c.check_tol >= 0.0))
+      return mlc_fail(MLC_EINVAL, "mlc_lac_create: bad configuration");
+    L.cfg = c;
+    const int K = g.nk;
+    L.nflp = (int)round_up(LAC_IN_FIELDS * K, 16);
+    L.nin = 9 * L.nflp;
+    L.nout = LAC_OUT_FIELDS * K;
+    L.rows.ntj = (g.nj + 63) / 64;
+    L.rtot = (long)L.rows.ntj * ((g.ni + 3) / 4) * 256;
+    L.chunk = (int)(c.chunk_rows < L.rtot ? c.chunk_rows : L.rtot);
+    int rc = mlp_new(L.nin, c.nhid, L.nout, c.seed, prm.device, stream, &L.net);
+    if (rc) return rc;
+#define AL(ptr, bytes) \
+  if ((rc = alloc_bytes((void **)&ptr, (size_t)(bytes))) != MLC_OK) return rc;
+    const size_t fT = (size_t)g.plane * K * sizeof(T), fT1 = (size_t)g.plane * (K + 1) * sizeof(T);
+    const size_t fD = (size_t)g.plane * K * sizeof(double);
+    AL(L.su, fT) AL(L.sv, fT) AL(L.sw, fT1) AL(L.stet, fT) AL(L.spai, fD);
+    AL(L.ru, fT) AL(L.rv, fT) AL(L.rw, fT1) AL(L.rpai, fD);
+    AL(L.cu, fT) AL(L.cv, fT) AL(L.cw, fT1) AL(L.cpai, fD);
+    AL(L.Y, (size_t)L.rtot * L.nout * sizeof(float));
+    AL(L.Xc, (size_t)L.chunk * L.nin * sizeof(__bf16));
+    AL(L.Xraw, (size_t)c.capacity * L.nin * sizeof(float));
+    AL(L.Traw, (size_t)c.capacity * L.nout * sizeof(float));
+    AL(L.cols, (size_t)c.samples_per_phase * sizeof(int));
+    AL(L.Craw, (size_t)c.capacity * sizeof(int));
+    AL(L.stat, (size_t)(2 * L.nin + 2 * L.nout) * sizeof(float));
+    AL(L.norms, 16 * sizeof(double));
+#undef AL
+    L.hcols.assign(c.samples_per_phase, 0);
+    L.rng = c.seed ^ 0xA5A5A5A55A5A5A5Aull;
+    L.exists = true;
+    HIP_TRY(hipStreamSynchronize(stream));
+    return MLC_OK;
+  }
+
+  template <typename F>
+  void copy_field(F *dst, const F *src, int nlev) {
+    const size_t n = (size_t)g.plane * nlev * sizeof(F);
+    timed(KK_LAC_COPY, 2.0 * g.nj * g.ni * nlev * sizeof(F),
+          [&]() { (void)hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, stream); });
+  }
+  // theta is only needed to restore the phase start (checks, collection).
+  void lac_snapshot(bool theta) {
+    copy_field(L.su, D.u, g.nk);
+    copy_field(L.sv, D.v, g.nk);
+    copy_field(L.sw, D.w, g.nk + 1);
+    if (theta) copy_field(L.stet, D.tetav, g.nk);
+    copy_field(L.spai, D.pai, g.nk);
+  }
+  void lac_restore() {
+    copy_field(D.u, L.su, g.nk);
+    copy_field(D.v, L.sv, g.nk);
+    copy_field(D.w, L.sw, g.nk + 1);
+    copy_field(D.tetav, L.stet, g.nk);
+    copy_field(D.pai, L.spai, g.nk);
+  }
+  void lac_save(T *u, T *v, T *w, double *pai) {
+    copy_field(u, D.u, g.nk);
+    copy_field(v, D.v, g.nk);
+    copy_field(w, D.w, g.nk + 1);
+    copy_field(pai, D.pai, g.nk);
+  }
+
+  LacSnap<T> lac_snap() const { return LacSnap<T>{L.su, L.sv, L.sw, L.spai}; }
+
+  // Network outputs for every row from the current state and the snapshot.
+  int lac_infer() {
+    const float *st = L.stat;
+    const double nh = L.cfg.nhid;
+    for (long r0 = 0; r0 < L.rtot; r0 += L.chunk) {
+      const int n = (int)(L.rtot - r0 < L.chunk ? L.rtot - r0 : L.chunk);
+      timed(KK_LAC_GATHER, (double)n * L.nin * sizeof(__bf16) + fbytes(6, 0) * n / L.rtot, [&]() {
+        hipLaunchKernelGGL(k_lac_gather<T>, dim3(n / 64), dim3(256), 0, stream, D, lac_snap(), L.rows,
+                           (const int *)nullptr, r0, n, L.nin, (float *)nullptr, L.Xc, st, st + L.nin);
+      });
+      int rc = MLC_OK;
+      timed(KK_LAC_MLP, 2.0 * n * (L.nin * nh + nh * nh + nh * L.nout),
+            [&]() { rc = mlp_forward_dev(L.net, L.Xc, L.nin, n, L.Y + r0 * L.nout, L.nout); });
+      if (rc) return rc;
+    }
+    return MLC_OK;
+  }
+
+  void lac_apply() {
+    const float *st = L.stat + 2 * L.nin;
+    launch(KK_LAC_APPLY, (double)L.rtot * L.nout * sizeof(float) + fbytes(6, 2), k_lac_apply<T>,
+           dim3((unsigned)(L.rtot / 64)), dim3(256), D, L.rows, (const float *)L.Y, st, st + L.nout);
+  }
+
+  // Squared norms of (a - b) and b for the four corrected fields into
+  // L.norms[4*f + 2*which ...]: which 0 = with the increment, 1 = without.
+  void lac_norms(int which, const T *u, const T *v, const T *w, const double *pai) {
+    double *o = L.norms + 2 * which;
+    launch(KK_DIAG, 0.0, k_diffnorm<T>, col_grid(), dim3(256), u, (const T *)D.u, g, 1, g.nk, o);
+    launch(KK_DIAG, 0.0, k_diffnorm<T>, col_grid(), dim3(256), v, (const T *)D.v, g, 1, g.nk, o + 4);
+    launch(KK_DIAG, 0.0, k_diffnorm<T>, col_grid(), dim3(256), w, (const T *)D.w, g, 2, g.nk, o + 8);
+    launch(KK_DIAG, 0.0, k_diffnorm<double>, col_grid(), dim3(256), pai, (const double *)D.pai, g, 1, g.nk, o + 12);
+  }
+
+  // One acoustic phase with the long substep.  correct: add the network's
+  // increment; check: also run the ordinary substeps from the same start,
+  // compare, and continue from the ordinary result.
+  int sound_lac(double dts, bool correct, bool check, double *worst) {
+    if (!prm.fulleq) halo(D.tetav, g.nk, 0);
+    const int nb = prm.nsound - 1;
+    int rc;
+    lac_snapshot(check);
+    substep(dts * nb, 1);
+    if (correct && (rc = lac_infer())) return rc;
+    if (check) lac_save(L.ru, L.rv, L.rw, L.rpai);
+    if (correct) lac_apply();
+    if (check) {
+      lac_save(L.cu, L.cv, L.cw, L.cpai);
+      lac_restore();
+      for (int ns = 1; ns <= nb; ns++) substep(dts, ns);
+      HIP_TRY(hipMemsetAsync(L.norms, 0, 16 * sizeof(double), stream));
+      lac_norms(0, L.cu, L.cv, L.cw, L.cpai);
+      lac_norms(1, L.ru, L.rv, L.rw, L.rpai);
+      double nr[16];
+      HIP_TRY(hipMemcpyAsync(nr, L.norms, sizeof(nr), hipMemcpyDeviceToHost, stream));
+      HIP_TRY(hipStreamSynchronize(stream));
+      for (int f = 0; f < 4; f++) {
+        const double ec = nr[4 * f], eu = nr[4 * f + 2], ref = nr[4 * f + 1];
+        const double ratio = eu > 0.0 ? std::sqrt(ec / eu) : (ec > 0.0 ? INFINITY : 0.0);
+        L.st.ratio[f] = ratio;
+        L.st.rel[f] = ref > 0.0 ? std::sqrt(ec / ref) : std::sqrt(ec);
+        if (!(ratio <= *worst)) *worst = ratio;
+      }
+      L.st.checks++;
+    }
+    substep(dts, prm.nsound);
+    if (check)
+      L.st.plain_phases++;
+    else if (correct)
+      L.st.learned_phases++;
+    return MLC_OK;
+  }
+
+  // One ordinary acoustic phase that also records training pairs at
+  // sampled columns; the state evolves exactly as in sound().
+  int sound_collect(double dts) {
+    if (!prm.fulleq) halo(D.tetav, g.nk, 0);
+    const int nb = prm.nsound - 1, ns = L.cfg.samples_per_phase;
+    const long ncol = (long)g.nj * g.ni;
+    for (int r = 0; r < ns; r++) L.hcols[r] = (int)(L.next_u64() % (uint64_t)ncol);
+    HIP_TRY(hipMemcpyAsync(L.cols, L.hcols.data(), ns * sizeof(int), hipMemcpyHostToDevice, stream));
+    if (L.next + ns > L.cfg.capacity) L.next = 0;
+    const long base = L.next;
+    const long rows_fit = L.cfg.capacity - base;
+    const int n = (int)(ns < rows_fit ? ns : rows_fit);
+    lac_snapshot(true);
+    substep(dts * nb, 1);
+    launch(KK_LAC_GATHER, 0.0, k_lac_gather<T>, dim3((n + 63) / 64), dim3(256), D, lac_snap(), L.rows,
+           (const int *)L.cols, 0L, n, L.nin, L.Xraw + base * L.nin, (__bf16 *)nullptr, (const float *)nullptr,
+           (const float *)nullptr);
+    lac_save(L.ru, L.rv, L.rw, L.rpai);
+    lac_restore();
+    for (int s = 1; s <= nb; s++) substep(dts, s);
+    launch(KK_LAC_GATHER, 0.0, k_lac_target<T>, dim3((unsigned)(((long)n * L.nout + 255) / 256)), dim3(256), D,
+           (const int *)L.cols, n, (const T *)L.ru, (const T *)L.rv, (const T *)L.rw, (const double *)L.rpai,
+           L.Traw + base * L.nout);
+    substep(dts, prm.nsound);
+    HIP_TRY(hipMemcpyAsync(L.Craw + base, L.cols, n * sizeof(int), hipMemcpyDeviceToDevice, stream));
+    L.next = base + n;
+    if (L.next > L.count) L.count = L.next;
+    L.st.samples = L.count;
+    L.st.plain_phases++;
+    HIP_TRY(hipStreamSynchronize(stream));  // hcols is reused by the next phase
+    return MLC_OK;
+  }
+
+  // Input normalisation per (field, level), pooled over the stencil
+  // positions; target normalisation per output.
+  int lac_normalise() {
+    const int nin = L.nin, nout = L.nout;
+    double *dm = nullptr;
+    HIP_TRY(hipMalloc((void **)&dm, (size_t)2 * (nin + nout) * sizeof(double)));
+    hipLaunchKernelGGL(k_colstats, dim3(nin), dim3(256), 0, stream, (const float *)L.Xraw, L.count, nin, dm,
+                       dm + nin + nout);
+    hipLaunchKernelGGL(k_colstats, dim3(nout), dim3(256), 0, stream, (const float *)L.Traw, L.count, nout, dm + nin,
+                       dm + 2 * nin + nout);
+    std::vector<double> h(2 * (nin + nout));
+    hipError_t e = hipMemcpyAsync(h.data(), dm, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(dm);
+    if (e != hipSuccess) return mlc_fail(MLC_EHIP, "lac normalisation: %s", hipGetErrorString(e));
+    const double *mean = h.data(), *sd = h.data() + nin + nout;
+    std::vector<double> fm(L.nflp, 0.0), fq(L.nflp, 0.0);
+    for (int o = 0; o < nin; o++) {
+      int fl, p;
+      lac_feature(o, fl, p);
+      fm[fl] += mean[o] / 9.0;
+      fq[fl] += (sd[o] * sd[o] + mean[o] * mean[o]) / 9.0;
+    }
+    std::vector<float> st(2 * nin + 2 * nout);
+    for (int o = 0; o < nin; o++) {
+      int fl, p;
+      lac_feature(o, fl, p);
+      const double var = fq[fl] - fm[fl] * fm[fl];
+      const double s = var > 0.0 ? std::sqrt(var) : 0.0;
+      st[o] = (float)fm[fl];
+      st[nin + o] = s > 1e-30 * (1.0 + std::fabs(fm[fl])) ? (float)(1.0 / s) : 0.f;
+    }
+    for (int o = 0; o < nout; o++) {
+      st[2 * nin + o] = (float)mean[nin + o];
+      const double s = sd[nin + o];
+      st[2 * nin + nout + o] = s > 0.0 ? (float)s : 1.f;
+    }
+    HIP_TRY(hipMemcpyAsync(L.stat, st.data(), st.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    L.normalised = true;
+    return MLC_OK;
+  }
+
+  int lac_train(int iters, int batch, double lr, double *loss) override {
+    if (!L.exists) return mlc_fail(MLC_ESTATE, "mlc_lac_train: no corrector (mlc_lac_create)");
+    if (iters < 0 || batch < 1 || !(lr > 0.0)) return mlc_fail(MLC_EINVAL, "mlc_lac_train: bad arguments");
+    if (L.count < 1) return mlc_fail(MLC_ESTATE, "mlc_lac_train: the training set is empty (mlc_lac_collect)");
+    int rc;
+    if (!L.normalised && (rc = lac_normalise())) return rc;
+    if (batch > L.bcap) {
+      if ((rc = alloc_bytes((void **)&L.Xb, (size_t)batch * L.nin * sizeof(__bf16)))) return rc;
+      if ((rc = alloc_bytes((void **)&L.Tn, (size_t)batch * L.nout * sizeof(float)))) return rc;
+      L.bcap = batch;
+    }
+    const long nidx = (long)iters * batch;
+    if (nidx > L.icap) {
+      if ((rc = alloc_bytes((void **)&L.bidx, (size_t)nidx * sizeof(int)))) return rc;
+      L.icap = nidx;
+    }
+    std::vector<int> idx(nidx);
+    for (long r = 0; r < nidx; r++) idx[r] = (int)(L.next_u64() % (uint64_t)L.count);
+    HIP_TRY(hipMemcpyAsync(L.bidx, idx.data(), nidx * sizeof(int), hipMemcpyHostToDevice, stream));
+    const float *st = L.stat;
+    double l = NAN;
+    const unsigned gx = (unsigned)((L.nin + 255) / 256);
+    for (int it = 0; it < iters; it++) {
+      hipLaunchKernelGGL(k_lac_batch, dim3(gx, batch), dim3(256), 0, stream, (const float *)L.Xraw,
+                         (const float *)L.Traw, (const int *)(L.bidx + (long)it * batch), L.nin, L.nout, st,
+                         st + L.nin, st + 2 * L.nin, st + 2 * L.nin + L.nout, L.Xb, L.Tn);
+      if ((rc = mlp_backward_dev(L.net, L.Xb, L.nin, L.Tn, batch, it == iters - 1 ? &l : nullptr))) return rc;
+      if ((rc = mlp_adam_dev(L.net, lr))) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(stream));
+    HIP_TRY(hipGetLastError());
+    if (iters > 0) L.st.last_loss = l;
+    if (loss) *loss = l;
+    L.st.fallback = 0;
+    return MLC_OK;
+  }
+
+  int lac_collect(int nsteps) override {
+    if (!L.exists) return mlc_fail(MLC_ESTATE, "mlc_lac_collect: no corrector (mlc_lac_create)");
+    if (nsteps < 0) return mlc_fail(MLC_EINVAL, "mlc_lac_collect: nsteps < 0");
+    for (int s = 0; s < nsteps; s++) {
+      int rc = run(0, nullptr, true);
+      if (rc) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(stream));
+    HIP_TRY(hipGetLastError());
+    return MLC_OK;
+  }
+
+  int lac_mode(int mode) override {
+    if (!L.exists) return mlc_fail(MLC_ESTATE, "mlc_lac_mode: no corrector (mlc_lac_create)");
+    if (mode != MLC_LAC_OFF && mode != MLC_LAC_ON && mode != MLC_LAC_REDUCED)
+      return mlc_fail(MLC_EINVAL, "mlc_lac_mode: unknown mode %d", mode);
+    if (mode == MLC_LAC_ON && !L.normalised) return mlc_fail(MLC_ESTATE, "mlc_lac_mode: the corrector is not trained");
+    L.mode = mode;
+    L.learned_steps = 0;
+    return MLC_OK;
+  }
+
+  int lac_stats(mlc_lac_stats *s) override {
+    if (!L.exists) return mlc_fail(MLC_ESTATE, "mlc_lac_stats: no corrector (mlc_lac_create)");
+    *s = L.st;
+    return MLC_OK;
+  }
+
+  mlc_mlp *lac_network() override { return L.exists ? L.net : nullptr; }
+
+  int lac_dataset(long first, int n, float *X, float *Tt, int *cols) override {
+    if (!L.exists) return mlc_fail(MLC_ESTATE, "mlc_lac_dataset: no corrector (mlc_lac_create)");
+    if (first < 0 || n < 0 || first + n > L.count) return mlc_fail(MLC_EINVAL, "mlc_lac_dataset: rows out of range");
+    if (X) HIP_TRY(hipMemcpyAsync(X, L.Xraw + first * L.nin, (size_t)n * L.nin * 4, hipMemcpyDeviceToHost, stream));
+    if (Tt)
+      HIP_TRY(hipMemcpyAsync(Tt, L.Traw + first * L.nout, (size_t)n * L.nout * 4, hipMemcpyDeviceToHost, stream));
+    if (cols) HIP_TRY(hipMemcpyAsync(cols, L.Craw + first, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    return MLC_OK;
+  }
+
   int step(int istep, int lprint, mlc_diag *diag) override {
     (void)istep;
+    return run(lprint, diag, false);
+  }
+
+  // collect: record training pairs in every acoustic phase (ordinary loop).
+  int run(int lprint, mlc_diag *diag, bool collect) {
     if (!initialised) return mlc_fail(MLC_ESTATE, "mlc_step before mlc_init");
     if (lprint && !diag) return mlc_fail(MLC_EINVAL, "mlc_step: lprint set but diag is NULL");
     const double dtstepa = prm.dt / (double)prm.nadv;
     const double dtsound = dtstepa / (double)prm.nsound;
+    // acoustic loop of this step
+    const bool learned = L.exists && L.mode == MLC_LAC_ON && !L.st.fallback && !collect;
+    const bool reduced = L.exists && L.mode == MLC_LAC_REDUCED && !collect;
+    if (learned && !L.normalised) return mlc_fail(MLC_ESTATE, "mlc_step: the corrector is not trained");
+    const bool check = learned && L.cfg.check_every > 0 && L.learned_steps % L.cfg.check_every == 0;
     Par P = par();
     launch(KK_PROLOGUE, fbytes(10, 1), k_prologue<T>, col_grid(), blk(), D, P, prm.filterpai ? 1 : 0,
            (prm.fulleq && prm.filtertheta) ? 1 : 0);
+    double worst = 0.0;
     for (int n = 1; n <= prm.nadv; n++) {
-      sound(dtsound);
+      int rc = MLC_OK;
+      if (collect)
+        rc = sound_collect(dtsound);
+      else if (learned || reduced)
+        rc = sound_lac(dtsound, learned, check, &worst);
+      else {
+        sound(dtsound);
+        if (L.exists) L.st.plain_phases++;
+      }
+      if (rc) return rc;
       advection(dtstepa);
+    }
+    if (learned) {
+      L.learned_steps++;
+      if (check && worst > L.cfg.check_tol) L.st.fallback = 1;
     }
     if (prm.filterpai) {  // :333-341
       launch(KK_FILTER_BRACKET, fbytes(0, 3), k_sub<double, double>, pt_grid(), blk(), D.pai, (const double *)D.pf, g);
@@ -1942,4 +2551,49 @@ extern "C" int mlc_case_load(mlc_handle *h, const mlc_case *c) {
   if ((rc = mlc_set_levels(h, MLC_ZITAH, zitah.data(), kz))) return rc;
   if ((rc = mlc_set_levels(h, MLC_FFILT, ffilt.data(), kz))) return rc;
   return mlc_init(h);
+}
+
+// ============================================== learned acoustic corrector
+extern "C" int mlc_lac_default_config(mlc_lac_config *c) {
+  if (!c) return mlc_fail(MLC_EINVAL, "mlc_lac_default_config: NULL");
+  c->nhid = 512;
+  c->samples_per_phase = 4096;
+  c->capacity = 131072;
+  c->chunk_rows = 65536;
+  c->check_every = 50;
+  c->check_tol = 1.0;
+  c->seed = 1;
+  return MLC_OK;
+}
+
+extern "C" int mlc_lac_create(mlc_handle *h, const mlc_lac_config *c) {
+  if (!h || !c) return mlc_fail(MLC_EINVAL, "mlc_lac_create: NULL argument");
+  return h->core->lac_create(*c);
+}
+
+extern "C" int mlc_lac_collect(mlc_handle *h, int nsteps) {
+  if (!h) return mlc_fail(MLC_EINVAL, "mlc_lac_collect: NULL handle");
+  return h->core->lac_collect(nsteps);
+}
+
+extern "C" int mlc_lac_train(mlc_handle *h, int iters, int batch, double lr, double *loss) {
+  if (!h) return mlc_fail(MLC_EINVAL, "mlc_lac_train: NULL handle");
+  return h->core->lac_train(iters, batch, lr, loss);
+}
+
+extern "C" int mlc_lac_mode(mlc_handle *h, int mode) {
+  if (!h) return mlc_fail(MLC_EINVAL, "mlc_lac_mode: NULL handle");
+  return h->core->lac_mode(mode);
+}
+
+extern "C" int mlc_lac_get_stats(mlc_handle *h, mlc_lac_stats *s) {
+  if (!h || !s) return mlc_fail(MLC_EINVAL, "mlc_lac_get_stats: NULL argument");
+  return h->core->lac_stats(s);
+}
+
+extern "C" mlc_mlp *mlc_lac_network(mlc_handle *h) { return h ? h->core->lac_network() : nullptr; }
+
+extern "C" int mlc_lac_dataset(mlc_handle *h, long first, int n, float *X, float *T, int *cols) {
+  if (!h) return mlc_fail(MLC_EINVAL, "mlc_lac_dataset: NULL handle");
+  return h->core->lac_dataset(first, n, X, T, cols);
 }

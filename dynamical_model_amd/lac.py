@@ -11,25 +11,43 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import check, load
+from ._lib import MLC_LAC_OFF, MLC_LAC_ON, MLC_LAC_REDUCED, LacConfig, LacStats, check, load
+
+MODES = {"off": MLC_LAC_OFF, "on": MLC_LAC_ON, "reduced": MLC_LAC_REDUCED}
 
 
 def nin_for(kz: int) -> int:
-    return 9 * 6 * kz
+    """Inputs of the corrector: 9 stencil columns x roundup(6 kz, 16)."""
+    return 9 * ((6 * kz + 15) // 16 * 16)
 
 
 def nout_for(kz: int) -> int:
     return 4 * kz
 
 
+def feature_index(kz: int):
+    """(fl, p) of every input: fl = field*kz + level-1 (fields u, v, w at
+    levels 2..kz+1, Exner, theta, qv; fl >= 6 kz is zero padding) and the
+    stencil position p, (dj, di) = (p%3 - 1, p//3 - 1)."""
+    o = np.arange(nin_for(kz))
+    c = o >> 3
+    cc = c % 18
+    return (c // 18) * 16 + (cc // 9) * 8 + (o & 7), cc % 9
+
+
 class MLP:
     """nin -> nhid -> nhid -> nout with ReLU, BF16 MFMA, FP32 master weights."""
 
-    def __init__(self, nin: int, nhid: int, nout: int, seed: int = 1, device: int = 0):
+    def __init__(self, nin: int, nhid: int, nout: int, seed: int = 1, device: int = 0, _borrowed=None):
         self.nin, self.nhid, self.nout = nin, nhid, nout
+        if _borrowed is not None:  # a network owned by a model handle
+            self._h = C.c_void_p(_borrowed)
+            self._owned = False
+            return
         h = C.c_void_p()
         check("mlc_mlp_create", load().mlc_mlp_create(nin, nhid, nout, seed, device, C.byref(h)))
         self._h = h
+        self._owned = True
 
     @property
     def handle(self):
@@ -76,7 +94,8 @@ class MLP:
 
     def close(self) -> None:
         if self._h is not None:
-            load().mlc_mlp_destroy(self._h)
+            if self._owned:
+                load().mlc_mlp_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -84,6 +103,59 @@ class MLP:
             self.close()
         except Exception:
             pass
+
+
+def default_config(**overrides) -> LacConfig:
+    c = LacConfig()
+    check("mlc_lac_default_config", load().mlc_lac_default_config(C.byref(c)))
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
+class Corrector:
+    """The learned acoustic corrector attached to a Moloch handle."""
+
+    def __init__(self, model, config: LacConfig | None = None, **overrides):
+        self.model = model
+        cfg = config if config is not None else default_config(**overrides)
+        check("mlc_lac_create", load().mlc_lac_create(model._h, C.byref(cfg)))
+        self.config = cfg
+
+    def collect(self, nsteps: int) -> None:
+        """nsteps ordinary steps that also record training pairs."""
+        check("mlc_lac_collect", load().mlc_lac_collect(self.model._h, int(nsteps)))
+
+    def train(self, iters: int, batch: int = 512, lr: float = 1e-3) -> float:
+        loss = C.c_double()
+        check("mlc_lac_train", load().mlc_lac_train(self.model._h, int(iters), int(batch), float(lr),
+                                                    C.byref(loss)))
+        return loss.value
+
+    def mode(self, mode: str) -> None:
+        check("mlc_lac_mode", load().mlc_lac_mode(self.model._h, MODES[mode]))
+
+    def stats(self) -> dict:
+        s = LacStats()
+        check("mlc_lac_get_stats", load().mlc_lac_get_stats(self.model._h, C.byref(s)))
+        return {"samples": s.samples, "learned_phases": s.learned_phases, "plain_phases": s.plain_phases,
+                "checks": s.checks, "fallback": bool(s.fallback), "last_loss": s.last_loss,
+                "ratio": list(s.ratio), "rel": list(s.rel)}
+
+    def dataset(self, first: int, n: int, kz: int):
+        """Rows of the training set: (X raw inputs, T targets, sampled columns)."""
+        X = np.zeros((n, nin_for(kz)), np.float32)
+        T = np.zeros((n, nout_for(kz)), np.float32)
+        cols = np.zeros(n, np.int32)
+        check("mlc_lac_dataset", load().mlc_lac_dataset(self.model._h, int(first), int(n), X.ctypes.data,
+                                                        T.ctypes.data, cols.ctypes.data))
+        return X, T, cols
+
+    def network(self, kz: int) -> MLP:
+        ptr = load().mlc_lac_network(self.model._h)
+        if not ptr:
+            raise RuntimeError("the handle has no corrector network")
+        return MLP(nin_for(kz), self.config.nhid, nout_for(kz), _borrowed=ptr)
 
 
 def gemm_bf16(a: np.ndarray, b: np.ndarray, a_t: bool = False, b_t: bool = False) -> np.ndarray:

@@ -227,8 +227,10 @@ typedef struct mlc_case {
   unsigned long long seed;  /* splitmix64 seed of terrain and noise           */
 } mlc_case;
 
-/* Defaults of the benchmark configurations 0..4 (grid, terrain, wind,
- * humidity, perturbations; see DESIGN notes in the package). */
+/* Defaults of the benchmark configurations 0..4: 0: 100 x 100 x 30, one
+ * 500 m hill; 1: 512 x 512 x 40, one hill, noise and a warm bubble; 2, 3, 4:
+ * 1536 x 1536 x 60, 512 x 512 x 40 and 2048 x 2048 x 60 with a 20 km top, 20
+ * random hills, a jet and 60 % relative humidity. */
 int mlc_case_config(int config, mlc_case *c);
 
 /* Copy the case's ztop, zh, a0 and dzita = ztop/kz into *p. */

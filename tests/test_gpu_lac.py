@@ -1,0 +1,247 @@
+"""The learned acoustic corrector on the GPU.
+
+The corrector replaces the first nsound-1 acoustic substeps of a phase by one
+substep of (nsound-1)*dts plus a learned increment.  Checked here:
+
+* its training pairs against the CPU oracle: the inputs are the increments
+  of the long substep (and theta, qv) on the 3 x 3 stencil, the targets the
+  difference between nsound-1 ordinary substeps and the long substep, both
+  computed by the oracle's own
+  acoustic loop (nompi/mod_moloch.F90:483-851) with nsound and dts set to
+  the two schedules;
+* that collecting pairs leaves the trajectory bit-identical to ordinary steps;
+* that the trained increment beats the uncorrected long substep (the gain);
+* that a failing check falls back to the ordinary loop, bit-identically.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import dynamical_model_amd as dm
+import oracle_lib
+from dynamical_model_amd import _lib
+from dynamical_model_amd.lac import Corrector, feature_index, nin_for, nout_for
+from parity import PROGNOSTIC, small_config2
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(case, **pover):
+    hs = dm.build_case(case)
+    params = dm.case_params(case, **pover)
+    m = dm.Moloch()
+    m.allocate_moloch(hs.jx, hs.iy, hs.kz, params)
+    m.load_state(hs)
+    m.init_moloch()
+    return m, hs, params
+
+
+def _fields(m, names=PROGNOSTIC):
+    m.sync()
+    return {n: m.get_field(n) for n in names}
+
+
+def _same_interior(fa, fb):
+    """Bitwise equality over the points a step defines (halo corners beyond
+    the 1-wide ring are never read and may hold stale copies)."""
+    for n in PROGNOSTIC:
+        assert np.array_equal(dm.interior(fa[n]), dm.interior(fb[n])), n
+
+
+def _relerr(a, b):
+    a, b = dm.interior(a), dm.interior(b)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_training_pairs_match_oracle():
+    case = small_config2(40, 36, 16)
+    m, hs, params = _model(case)
+    spp = 300
+    corr = Corrector(m, nhid=64, samples_per_phase=spp, capacity=4 * spp, chunk_rows=1024, check_every=0, seed=7)
+    print(f"case 40x36x16 fp64, collect 1 step, {spp} samples per phase", flush=True)
+    corr.collect(1)
+    assert corr.stats()["samples"] == 3 * spp
+    X, T, cols = corr.dataset(0, spp, case.kz)  # the first acoustic phase
+    m.close()
+
+    kz, jx, iy = case.kz, case.jx, case.iy
+    dts = params.dt / params.nadv / params.nsound
+    nb = params.nsound - 1
+
+    def phase_start():
+        o = oracle_lib.oracle_from_host(hs, params)
+        o.call("ora_reset_tendencies")
+        o.call("ora_prologue")
+        return o
+
+    start, long_, plain = phase_start(), phase_start(), phase_start()
+    long_.S.nsound = 1
+    long_.call("ora_sound", nb * dts)
+    plain.S.nsound = nb
+    plain.call("ora_sound", dts)
+
+    j = cols % jx + 1
+    i = cols // jx + 1
+    # inputs: increments of the long substep, then theta and qv after it
+    src = [long_.field(n) - start.field(n) for n in ("u", "v", "w", "pai")]
+    src[2] = src[2][1:]
+    src += [long_.field("tetav"), long_.field("qv")]
+    fl, p = feature_index(kz)
+    dj, di = p % 3 - 1, p // 3 - 1
+    f, lev = fl // kz, fl % kz
+    Xo = np.zeros((spp, nin_for(kz)), np.float32)
+    for r in range(spp):
+        jj = (j[r] + dj - 1) % jx + 1
+        ii = (i[r] + di - 1) % iy + 1
+        val = np.zeros(fl.size)
+        ok = f < 6
+        for ff in range(6):
+            sel = ok & (f == ff)
+            val[sel] = src[ff][lev[sel], ii[sel] + 1, jj[sel] + 1]
+        Xo[r] = val.astype(np.float32)
+    assert np.array_equal(X, Xo)
+    # targets
+    To = np.zeros((spp, nout_for(kz)))
+    for ff, name in enumerate(["u", "v", "w", "pai"]):
+        a, b = plain.field(name), long_.field(name)
+        if name == "w":
+            a, b = a[1:], b[1:]
+        To[:, ff * kz:(ff + 1) * kz] = (a - b)[:, i + 1, j + 1].T
+    for ff, name in enumerate(["u", "v", "w", "pai"]):
+        t, to = T[:, ff * kz:(ff + 1) * kz], To[:, ff * kz:(ff + 1) * kz]
+        err = np.abs(t - to).max() / np.abs(to).max()
+        print(f"target {name}: max |T| {np.abs(to).max():.3e} rel err {err:.2e}")
+        assert np.abs(to).max() > 0, name
+        assert err < 1e-6, name
+
+
+@pytest.mark.parametrize("precision", [dm.MLC_FP64, dm.MLC_FP32])
+def test_collect_is_bitwise_ordinary(precision):
+    case = small_config2(48, 40, 16)
+    a, _, _ = _model(case, precision=precision)
+    b, _, _ = _model(case, precision=precision)
+    corr = Corrector(a, nhid=64, samples_per_phase=256, capacity=4096, chunk_rows=1024, check_every=0)
+    print(f"case 48x40x16 precision={precision}: collect 3 steps vs 3 ordinary steps", flush=True)
+    corr.collect(3)
+    for n in range(1, 4):
+        b.moloch(n)
+    _same_interior(_fields(a), _fields(b))
+    st = corr.stats()
+    assert st["samples"] == 3 * 3 * 256 and st["plain_phases"] == 9
+    a.close()
+    b.close()
+
+
+def _train(corr, iters):
+    losses = [corr.train(iters // 2, 256, 1e-3), corr.train(iters // 4, 256, 3e-4),
+              corr.train(iters // 4, 256, 1e-4)]
+    print("losses", " ".join(f"{x:.4f}" for x in losses), flush=True)
+    return losses
+
+
+@pytest.mark.parametrize("precision", [dm.MLC_FP32, dm.MLC_FP64])
+def test_learned_increment_beats_long_substep(precision):
+    case = small_config2(64, 64, 16)
+    learned, _, _ = _model(case, precision=precision)
+    ordinary, _, _ = _model(case, precision=precision)
+    reduced, _, _ = _model(case, precision=precision)
+    cfg = dict(nhid=256, samples_per_phase=2048, capacity=65536, chunk_rows=4096, check_every=0, seed=3)
+    corr = Corrector(learned, **cfg)
+    rcorr = Corrector(reduced, **cfg)
+    nwarm = 8
+    print(f"case 64x64x16 precision={precision}: collect {nwarm} steps", flush=True)
+    corr.collect(nwarm)
+    for n in range(1, nwarm + 1):
+        ordinary.moloch(n)
+        reduced.moloch(n)
+    losses = _train(corr, 2000)
+    assert losses[-1] < 0.5
+    corr.mode("on")
+    rcorr.mode("reduced")
+    for n in range(nwarm + 1, nwarm + 5):
+        learned.moloch(n)
+        ordinary.moloch(n)
+        reduced.moloch(n)
+    fl, fo, fr = _fields(learned), _fields(ordinary), _fields(reduced)
+    st = corr.stats()
+    assert st["learned_phases"] == 12 and not st["fallback"]
+    gains = {}
+    for n in ["u", "v", "w", "pai"]:
+        el, er = _relerr(fl[n], fo[n]), _relerr(fr[n], fo[n])
+        gains[n] = er / el
+        print(f"{n:4s} learned {el:.3e}  long substep {er:.3e}  gain {er / el:.2f}")
+        assert np.isfinite(fl[n]).all()
+    for n in ["u", "v", "w", "pai"]:
+        assert gains[n] > 1.5, (n, gains)
+    for m in (learned, ordinary, reduced):
+        m.close()
+
+
+def test_failed_check_falls_back_bitwise():
+    case = small_config2(48, 40, 16)
+    a, _, _ = _model(case, precision=dm.MLC_FP32)
+    b, _, _ = _model(case, precision=dm.MLC_FP32)
+    corr = Corrector(a, nhid=64, samples_per_phase=512, capacity=4096, chunk_rows=1024, check_every=1,
+                     check_tol=0.0)
+    print("case 48x40x16 fp32: check tolerance 0 forces the fallback", flush=True)
+    corr.collect(2)
+    for n in (1, 2):
+        b.moloch(n)
+    corr.train(20, 128, 1e-3)
+    corr.mode("on")
+    for n in range(3, 6):
+        a.moloch(n)
+        b.moloch(n)
+    _same_interior(_fields(a), _fields(b))
+    st = corr.stats()
+    print(st)
+    assert st["fallback"] and st["checks"] == 3 and st["learned_phases"] == 0
+    assert all(r > 0 for r in st["ratio"])
+    # retraining clears the fallback
+    corr.train(1, 128, 1e-4)
+    assert not corr.stats()["fallback"]
+    a.close()
+    b.close()
+
+
+def test_checks_pass_with_loose_tolerance():
+    case = small_config2(48, 40, 16)
+    a, _, _ = _model(case, precision=dm.MLC_FP32)
+    corr = Corrector(a, nhid=64, samples_per_phase=512, capacity=4096, chunk_rows=1024, check_every=2,
+                     check_tol=1e30)
+    print("case 48x40x16 fp32: checks every 2 steps, loose tolerance", flush=True)
+    corr.collect(2)
+    corr.train(50, 128, 1e-3)
+    corr.mode("on")
+    for n in range(3, 8):
+        a.moloch(n)
+    st = corr.stats()
+    print(st)
+    # learned steps 0, 2, 4 are checked (3 phases each), the others use the increment
+    assert st["checks"] == 9 and st["learned_phases"] == 6 and not st["fallback"]
+    assert all(np.isfinite(st["rel"]))
+    net = corr.network(case.kz)
+    w, b = net.get_weights(0)
+    assert w.shape == (nin_for(case.kz), 64) and np.isfinite(w).all()
+    a.close()
+
+
+def test_corrector_errors():
+    case = small_config2(32, 32, 8)
+    a, _, _ = _model(case, nsound=1)
+    with pytest.raises(dm.MolochError) as e:
+        Corrector(a, nhid=32, samples_per_phase=16, capacity=64, chunk_rows=256)
+    assert e.value.code == _lib.MLC_EINVAL
+    a.close()
+    b, _, _ = _model(case)
+    corr = Corrector(b, nhid=32, samples_per_phase=16, capacity=64, chunk_rows=256)
+    with pytest.raises(dm.MolochError) as e:
+        corr.mode("on")  # not trained
+    assert e.value.code == _lib.MLC_ESTATE
+    with pytest.raises(dm.MolochError) as e:
+        corr.train(1, 8, 1e-3)  # nothing collected
+    assert e.value.code == _lib.MLC_ESTATE
+    with pytest.raises(dm.MolochError):
+        Corrector(b, nhid=32, samples_per_phase=16, capacity=64, chunk_rows=256)  # second corrector
+    b.close()
