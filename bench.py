@@ -3,9 +3,12 @@
 
 Metric: column-steps/s per GPU -- one (j,i) column advanced by one
 moloch(istep, lprint) call (dt = 30 s: 3 advection cycles x 5 acoustic
-substeps, nompi/mod_moloch.F90:272-421) -- for the plain FP64 core and the
-FP32 core (FP64 Exner function, hydrostatic residuals and column solve), with
-the FP32 core's relative L2 error against the FP64 core after the same steps.
+substeps, nompi/mod_moloch.F90:272-421) -- for the plain FP64 core, the FP32
+core (FP64 Exner function, hydrostatic residuals and column solve) and the
+FP32 core with the learned acoustic corrector (one long acoustic substep plus
+a BF16 MFMA network increment, then the last ordinary substep; the network is
+trained on the GPU from ordinary steps of the same run before timing), with
+each variant's relative L2 error against the FP64 core after the same steps.
 
 Workload (default): configuration 4, 2048 x 2048 x 60, synthetic initial state
 (standard atmosphere, 20 seeded Gaussian hills, jet, RH 60 %, theta noise)
@@ -36,29 +39,70 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_variant(dm, hs, case, precision: int, steps: int, warmup: int):
+def run_variant(dm, hs, case, precision: int, steps: int, warmup: int, lead: int, learned: dict | None = None):
+    """lead ordinary steps (the corrector's training run when learned), warmup
+    untimed steps, then `steps` timed steps."""
     params = dm.case_params(case, precision=precision)
     m = dm.Moloch()
     m.allocate_moloch(hs.jx, hs.iy, hs.kz, params)
     m.load_state(hs)
     m.init_moloch()
-    for n in range(1, warmup + 1):
+    extra = {}
+    if learned is not None:
+        from dynamical_model_amd.lac import Corrector
+
+        corr = Corrector(m, **learned["config"])
+        t0 = time.perf_counter()
+        corr.collect(lead)
+        m.sync()
+        t1 = time.perf_counter()
+        losses = [corr.train(it, learned["batch"], lr) for it, lr in learned["schedule"]]
+        t2 = time.perf_counter()
+        corr.mode("on")
+        extra = {"collect_s": t1 - t0, "train_s": t2 - t1, "losses": losses,
+                 "samples": corr.stats()["samples"]}
+        log(f"corrector: collected {extra['samples']} pairs in {t1 - t0:.1f} s, trained in {t2 - t1:.1f} s, "
+            f"losses {' '.join(f'{x:.4f}' for x in losses)}")
+    else:
+        for n in range(1, lead + 1):
+            m.moloch(n, False)
+    for n in range(lead + 1, lead + warmup + 1):
         m.moloch(n, False)
     m.sync()
     m.profile(True)
     m.sync()
     t0 = time.perf_counter()
-    for n in range(warmup + 1, warmup + steps + 1):
+    first = lead + warmup + 1
+    for n in range(first, first + steps):
         m.moloch(n, False)
     m.sync()
     t1 = time.perf_counter()
     prof = m.profile_report()
     m.profile(False)
-    diag = m.moloch(warmup + steps + 1, True)  # one more step with the lprint diagnostics
+    diag = m.moloch(first + steps, True)  # one more step with the lprint diagnostics
     m.sync()
+    if learned is not None:
+        extra["stats"] = corr.stats()
     finals = {f: dm.interior(m.get_field(f)).copy() for f in PROGNOSTIC}
     m.close()
-    return (t1 - t0), prof, finals, diag
+    return (t1 - t0), prof, finals, diag, extra
+
+
+MFMA_KINDS = {"lac_mlp"}  # work counted in flops; every other kind in bytes
+MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
+
+
+def roofline_of(prof: dict, name: str) -> dict:
+    ms, n, work = prof[name]
+    if name in MFMA_KINDS:
+        achieved = work / (ms * 1e-3) / 1e12
+        return {"bound": "mfma", "achieved": achieved, "peak": MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / MFMA_PEAK_TFLOPS, "traffic": None, "kernel": name,
+                "flops_per_launch": work / n, "avg_launch_ms": ms / n}
+    achieved = work / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name,
+            "bytes_per_launch": work / n, "avg_launch_ms": ms / n}
 
 
 def dominant(prof: dict) -> tuple[str, float, float, int]:
@@ -98,6 +142,12 @@ def main() -> int:
     ap.add_argument("--iy", type=int, default=0)
     ap.add_argument("--kz", type=int, default=0)
     ap.add_argument("--cpu-sample", default="128x128x4", help="JXxIYxSTEPS of the CPU oracle sample, or 'none'")
+    ap.add_argument("--lead", type=int, default=6,
+                    help="ordinary steps before warm-up (the corrector's training run)")
+    ap.add_argument("--train-iters", type=int, default=1600, help="Adam steps of the corrector")
+    ap.add_argument("--variants", default="fp64,fp32,learned")
+    ap.add_argument("--check-tol", type=float, default=1.0,
+                    help="corrector check: fall back when a field's error ratio exceeds this")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -114,35 +164,48 @@ def main() -> int:
     hs = dm.build_case(case)
     log(f"built configuration {args.config} ({jx}x{iy}x{kz}) on the host in {time.perf_counter() - t0:.1f} s")
 
+    learned_cfg = {
+        "config": dict(nhid=512, samples_per_phase=4096, capacity=131072, chunk_rows=65536, check_every=50,
+                       check_tol=args.check_tol, seed=11, engine=1),
+        "batch": 512,
+        "schedule": [(args.train_iters // 2, 1e-3), (args.train_iters // 4, 3e-4), (args.train_iters // 4, 1e-4)],
+    }
+    variants = {"fp64": (dm.MLC_FP64, None), "fp32": (dm.MLC_FP32, None), "learned": (dm.MLC_FP32, learned_cfg)}
+    wanted = [v for v in args.variants.split(",") if v]
+    if "fp64" not in wanted:
+        wanted.insert(0, "fp64")  # the reference of the error figures
     results = {}
     finals = {}
-    for tag, prec in (("fp64", dm.MLC_FP64), ("fp32", dm.MLC_FP32)):
+    for tag in wanted:
+        prec, lcfg = variants[tag]
         t0 = time.perf_counter()
-        secs, prof, fin, diag = run_variant(dm, hs, case, prec, args.steps, args.warmup)
+        secs, prof, fin, diag, extra = run_variant(dm, hs, case, prec, args.steps, args.warmup, args.lead, lcfg)
         cs = jx * iy * args.steps / secs
         name, ms, b, n = dominant(prof)
         results[tag] = {
             "value": cs, "ms_per_step": 1e3 * secs / args.steps, "dominant_kernel": name,
-            "dominant_ms_avg": ms / n, "dominant_gbs": b / (ms * 1e-3) / 1e9,
-            "kernels": {k: {"ms_total": v[0], "launches": v[1], "gbs": (v[2] / (v[0] * 1e-3) / 1e9) if v[0] else 0.0}
+            "kernels": {k: {"ms_total": v[0], "launches": v[1],
+                            "work_per_s": (v[2] / (v[0] * 1e-3)) if v[0] else 0.0}
                         for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
             "diag": {"ps_max": diag.ps_max, "ps_min": diag.ps_min, "u_max": diag.u_max, "u_min": diag.u_min,
                      "v_max": diag.v_max, "v_min": diag.v_min, "cfl_violation": diag.cfl_violation},
             "_prof": prof,
         }
+        if extra:
+            results[tag]["corrector"] = extra
         finals[tag] = fin
         log(f"{tag}: {cs:.4g} column-steps/s, {1e3 * secs / args.steps:.2f} ms/step, dominant {name} "
-            f"{results[tag]['dominant_gbs']:.0f} GB/s ({time.perf_counter() - t0:.1f} s incl. set-up)")
+            f"({time.perf_counter() - t0:.1f} s incl. set-up)")
 
-    errors = {f: rel_l2(finals["fp32"][f], finals["fp64"][f]) for f in PROGNOSTIC}
+    errors = {tag: {f: rel_l2(finals[tag][f], finals["fp64"][f]) for f in PROGNOSTIC}
+              for tag in results if tag != "fp64"}
     del finals
 
-    chosen = results["fp32"]
-    name, ms, b, n = dominant(chosen["_prof"])
-    achieved = b / (ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name,
-                "bytes_per_launch": b / n, "avg_launch_ms": ms / n}
+    chosen_tag = "learned" if "learned" in results else ("fp32" if "fp32" in results else "fp64")
+    chosen = results[chosen_tag]
+    roofline = roofline_of(chosen["_prof"], dominant(chosen["_prof"])[0])
+    stencil = roofline_of(chosen["_prof"], max((k for k in chosen["_prof"] if k not in MFMA_KINDS),
+                                               key=lambda k: chosen["_prof"][k][0]))
 
     cpu = None
     if args.cpu_sample != "none":
@@ -155,6 +218,8 @@ def main() -> int:
 
     for r in results.values():
         r.pop("_prof")
+    if "learned" in results:
+        results["learned"]["corrector"]["stats"] = dict(results["learned"]["corrector"].get("stats", {}))
     line = {
         "metric": METRIC,
         "value": chosen["value"],
@@ -166,17 +231,22 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" + ("+bf16" if chosen_tag == "learned" else ""),
         "data": "synthetic",
         "config": {"workload": f"configuration {args.config}: MOLOCH step on {jx}x{iy}x{kz} "
-                               "(dt=30 s, 3 advection cycles x 5 acoustic substeps), FP32 state with FP64 "
-                               "Exner function, hydrostatic residuals and tridiagonal solve",
-                   "jx": jx, "iy": iy, "kz": kz, "precision": "fp32+fp64-solve"},
+                               "(dt=30 s, 3 advection cycles x 5 acoustic substeps); "
+                               + {"learned": "FP32 state, FP64 Exner/solve, learned acoustic corrector "
+                                             "(long substep + BF16 MFMA MLP increment)",
+                                  "fp32": "FP32 state, FP64 Exner function, hydrostatic residuals and solve",
+                                  "fp64": "FP64"}[chosen_tag],
+                   "jx": jx, "iy": iy, "kz": kz, "variant": chosen_tag,
+                   "lead_steps": args.lead},
         "fp64": {"value": results["fp64"]["value"], "ms_per_step": results["fp64"]["ms_per_step"], "dtype": "f64"},
-        "speedup_vs_fp64": chosen["value"] / results["fp64"]["value"],
+        "speedup_vs_fp64": {t: r["value"] / results["fp64"]["value"] for t, r in results.items()},
         "rel_l2_vs_fp64": errors,
-        "rel_l2_after_steps": args.warmup + args.steps + 1,
+        "rel_l2_after_steps": args.lead + args.warmup + args.steps + 1,
         "roofline": roofline,
+        "stencil_roofline": stencil,
         "cpu_baseline": cpu,
         "variants": results,
     }

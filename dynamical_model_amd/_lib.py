@@ -84,7 +84,7 @@ class LacConfig(C.Structure):
     _fields_ = [
         ("nhid", C.c_int), ("samples_per_phase", C.c_int), ("capacity", C.c_int),
         ("chunk_rows", C.c_int), ("check_every", C.c_int), ("check_tol", C.c_double),
-        ("seed", C.c_ulonglong),
+        ("seed", C.c_ulonglong), ("engine", C.c_int),
     ]
 
 
@@ -93,6 +93,7 @@ class LacStats(C.Structure):
         ("samples", C.c_longlong), ("learned_phases", C.c_longlong),
         ("plain_phases", C.c_longlong), ("checks", C.c_longlong), ("fallback", C.c_int),
         ("last_loss", C.c_double), ("ratio", C.c_double * 4), ("rel", C.c_double * 4),
+        ("worst_ratio", C.c_double),
     ]
 
 

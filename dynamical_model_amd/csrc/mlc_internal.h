@@ -27,6 +27,13 @@ struct mlc_mlp {
   __bf16 *H1 = nullptr, *H2 = nullptr, *dH1 = nullptr, *dH2 = nullptr, *dYb = nullptr;
   float *Y = nullptr, *dYf = nullptr, *dH1f = nullptr, *dH2f = nullptr;
   double *partial = nullptr;
+  // weights packed as MFMA B fragments for the fused inference kernels:
+  // fragment (ks, nt) holds 64 lanes x 8 bf16, lane = n%32 + 32*((k%16)/8),
+  // element e = k%8, for k-step ks = k/16 and n-tile nt = n/32; the n extent
+  // is padded to a multiple of 256 and k to a multiple of 16 with zeros.
+  __bf16 *Wp[3] = {};
+  int wp_nt[3] = {}, wp_ks[3] = {};
+  bool pack_dirty = true;
   std::vector<void *> allocs;
   int in_dim(int l) const { return l == 0 ? nin : nhid; }
   int out_dim(int l) const { return l == 2 ? nout : nhid; }
@@ -44,3 +51,7 @@ int mlp_forward_dev(mlc_mlp *p, const __bf16 *X, long ldx, int rows, float *Yout
 int mlp_backward_dev(mlc_mlp *p, const __bf16 *X, long ldx, const float *T, int rows, double *loss);
 // One Adam step with the gradients of the last backward.
 int mlp_adam_dev(mlc_mlp *p, double lr);
+// Make room for activations of `rows` rows (H1, H2, ...).
+int mlp_reserve_rows(mlc_mlp *p, int rows);
+// Refresh the packed weight fragments if the weights changed.
+int mlp_pack_dev(mlc_mlp *p);

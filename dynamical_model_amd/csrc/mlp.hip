@@ -204,6 +204,21 @@ __global__ void k_adam(float *w, __bf16 *wb, const float *g, float *m, float *v,
   if (wb) wb[t] = to_bf16(wt);
 }
 
+// Pack bf16 weights W [in][out] into MFMA B fragments (layout in
+// mlc_internal.h): one thread per fragment lane.
+__global__ void k_pack_frag(const __bf16 *W, int in, int out, int nt_total, long nfrag_lanes, __bf16 *P) {
+  long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= nfrag_lanes) return;
+  const int lane = (int)(t & 63);
+  const long frag = t >> 6;
+  const int nt = (int)(frag % nt_total), ks = (int)(frag / nt_total);
+  const int n = nt * 32 + (lane & 31), k0 = ks * 16 + 8 * (lane >> 5);
+  for (int e = 0; e < 8; e++) {
+    const int k = k0 + e;
+    P[t * 8 + e] = (k < in && n < out) ? W[(long)k * out + n] : to_bf16(0.f);
+  }
+}
+
 }  // namespace mlc
 
 using namespace mlc;
@@ -219,7 +234,10 @@ static int mlp_alloc(mlc_mlp *p, void **ptr, size_t bytes) {
   return MLC_OK;
 }
 
-static int mlp_reserve(mlc_mlp *p, int rows) {
+int mlp_reserve_rows(mlc_mlp *p, int rows);
+static int mlp_reserve(mlc_mlp *p, int rows) { return mlp_reserve_rows(p, rows); }
+
+int mlp_reserve_rows(mlc_mlp *p, int rows) {
   if (rows <= p->cap) return MLC_OK;
   for (void *q : {(void *)p->H1, (void *)p->H2, (void *)p->dH1, (void *)p->dH2, (void *)p->dYb, (void *)p->Y,
                   (void *)p->dYf, (void *)p->dH1f, (void *)p->dH2f, (void *)p->partial}) {
@@ -321,8 +339,29 @@ int mlp_backward_dev(mlc_mlp *p, const __bf16 *X, long ldx, const float *T, int 
   return MLC_OK;
 }
 
+int mlp_pack_dev(mlc_mlp *p) {
+  if (!p->pack_dirty) return MLC_OK;
+  for (int l = 0; l < 3; l++) {
+    const int in = p->in_dim(l), out = p->out_dim(l);
+    const int nt_total = ((out + 255) / 256) * 8, ks_total = (in + 15) / 16;
+    const long lanes = (long)nt_total * ks_total * 64;
+    if (!p->Wp[l]) {
+      int rc = mlp_alloc(p, (void **)&p->Wp[l], lanes * 8 * sizeof(__bf16));
+      if (rc) return rc;
+      p->wp_nt[l] = nt_total;
+      p->wp_ks[l] = ks_total;
+    }
+    hipLaunchKernelGGL(k_pack_frag, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, p->stream, p->Wb[l], in, out,
+                       nt_total, lanes, p->Wp[l]);
+  }
+  MLP_TRY(hipGetLastError());
+  p->pack_dirty = false;
+  return MLC_OK;
+}
+
 int mlp_adam_dev(mlc_mlp *p, double lr) {
   p->step++;
+  p->pack_dirty = true;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   const float c1 = 1.f - powf(b1, (float)p->step), c2 = 1.f - powf(b2, (float)p->step);
   for (int l = 0; l < 3; l++) {
@@ -337,6 +376,7 @@ int mlp_adam_dev(mlc_mlp *p, double lr) {
 }
 
 static int mlp_sync_bf16(mlc_mlp *p) {
+  p->pack_dirty = true;
   for (int l = 0; l < 3; l++) {
     long nw = (long)p->in_dim(l) * p->out_dim(l);
     hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, p->stream, p->W[l],

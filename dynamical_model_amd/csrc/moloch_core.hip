@@ -1211,6 +1211,229 @@ __global__ void __launch_bounds__(256) k_lac_gather(Dev<T> D, LacSnap<T> S, LacR
   }
 }
 
+// ------------------------------------------------ fused network inference
+// Inputs of the whole grid are first written as normalised bf16 "feature
+// planes" F[pair dg][column][16 fl] (a pair of fl groups per plane, 32
+// bytes per column), over the columns of the row tiles plus a periodic
+// one-column ring: extended column (jj, ii), jj in [0, fpj), ii in [0, fpi),
+// is grid column (wrap(jj), wrap(ii)).  The values are the training
+// gather's, bit for bit.
+struct FeatPlanes {
+  __bf16 *F;
+  int fpj, fpi;  // extended columns along j and i
+  __host__ __device__ long plane() const { return (long)fpj * fpi * 16; }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_lac_features(Dev<T> D, LacSnap<T> S, FeatPlanes P, int ndg,
+                                                      const float *fmean, const float *frstd) {
+  typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf8;
+  const int jj = blockIdx.x * 64 + threadIdx.x % 64, ii = blockIdx.y * 4 + threadIdx.x / 64;
+  if (jj >= P.fpj || ii >= P.fpi) return;
+  const long col = (long)ii * P.fpj + jj;
+  const int j = ((jj - 1) % D.g.nj + D.g.nj) % D.g.nj + 1, i = ((ii - 1) % D.g.ni + D.g.ni) % D.g.ni + 1;
+  for (int dg = 0; dg < ndg; dg++) {
+    bf8 h[2];
+#pragma unroll
+    for (int fi = 0; fi < 16; fi++) {
+      const int fl = dg * 16 + fi;
+      const float v = lac_value(D, S, fl, j, i);
+      h[fi >> 3][fi & 7] = (__bf16)((v - fmean[fl]) * frstd[fl]);
+    }
+    bf8 *dst = reinterpret_cast<bf8 *>(P.F + dg * P.plane() + col * 16);
+    dst[0] = h[0];
+    dst[1] = h[1];
+  }
+}
+
+// Block tile of 256 rows x 256 outputs on v_mfma_f32_32x32x16_bf16: eight
+// waves (two per SIMD, so one wave's MFMAs cover the other's memory waits)
+// of 128 rows x 64 outputs each.  B comes from weight fragments packed in
+// the MFMA lane order, loaded DB k-steps ahead into registers and staged
+// through LDS one k-step (16) at a time.  A is either
+//  * the feature planes (layer 1): the block's 256 rows are one 64 x 4 tile
+//    of columns; for each pair of fl groups (16 fl, 9 stencil positions = 9
+//    k-steps) the 66 x 6 halo of the tile is copied into LDS as [column][16
+//    fl] and every stencil position reads it at its offset; the next pair is
+//    loaded when the current one starts;
+//  * row-major bf16 activations (layers 2 and 3), loaded DB k-steps ahead.
+// Epilogue: bias, then ReLU into bf16 H, or plain fp32 into Y.
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 lac_bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float lac_f32x16;
+
+constexpr int FM = 256, FN = 256, FT = 512;  // block tile, threads
+constexpr int HALO_J = 66, HALO_I = 6, HALO_C = HALO_J * HALO_I;
+constexpr int ACOL = 24;  // bf16 per LDS column / row (16 used): 48-byte stride, conflict-free 16-byte reads
+constexpr int DB = 4;     // k-steps of register prefetch
+static_assert(HALO_C <= FT, "one halo column per thread");
+
+struct FusedArgs {
+  const lac_bf16x8 *Bp;  // packed fragments
+  int bnt;               // n-tiles of 32 in the packed layout
+  int nks;               // k-steps of 16
+  int N;                 // valid outputs
+  const float *bias;
+  const __bf16 *A;       // row-major activations (rows of this launch), or null
+  long lda;
+  __bf16 *H;             // bf16 output with ReLU (ldh), or null
+  long ldh;
+  float *Y;              // fp32 output (ldy) when H is null
+  long ldy;
+  long row0;             // tile-order row of the launch's first row
+  int nrt;               // row tiles (of 256) in this launch
+  int nN;                // output tiles (of 256)
+  FeatPlanes fp;         // layer 1 inputs
+};
+
+template <bool IMPLICIT>
+__global__ void __launch_bounds__(FT) k_mlp_fused(LacRows R, FusedArgs a) {
+  // The k loop is unrolled in groups of KG steps (9 = one pair of fl groups
+  // for the feature planes, 4 for activations) with prefetch depth PD
+  // dividing KG, so register slots are compile-time and the loop body is
+  // free of branches: prefetches past the end are clamped to the last stage
+  // and land in buffers nobody reads any more.  Threads without a halo
+  // column copy into spare LDS columns.
+  constexpr int KG = IMPLICIT ? 9 : 4, PD = IMPLICIT ? 3 : 4;
+  constexpr int ACOLS = IMPLICIT ? FT : FM;  // LDS columns (halo + spare) or rows
+  __shared__ __attribute__((aligned(16))) __bf16 sA[2][ACOLS * ACOL];
+  __shared__ __attribute__((aligned(16))) lac_bf16x8 sB[2][FN / 32][64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 2, wn = wave & 3;  // rows wm*128, outputs wn*64
+  // XCD-aware order: consecutive work items on the same XCD (round-robin
+  // dispatch), so the output tiles of one row tile and neighbouring row
+  // tiles share that XCD's L2.
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int lin = (b & 7) * (nb >> 3) + (b >> 3);
+  const int rt = lin / a.nN, ntile = lin % a.nN;
+  if (rt >= a.nrt) return;
+  const long rowb = a.row0 + (long)rt * FM;
+  const int n0 = ntile * FN;
+  const int nks = a.nks;  // a multiple of KG
+
+  lac_f32x16 acc[4][2];
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[x][y][r] = 0.f;
+
+  // B of k-step ks: 8 fragments x 64 lanes, one per thread
+  const lac_bf16x8 *bsrc = a.Bp + ((long)(n0 >> 5) + (tid >> 6)) * 64 + lane;
+  lac_bf16x8 breg[PD], areg[PD];
+  auto loadB = [&](int ks, int slot) {
+    ks = ks < nks ? ks : nks - 1;
+    breg[slot] = bsrc[(long)ks * a.bnt * 64];
+  };
+  auto commitB = [&](int slot, int buf) { sB[buf][tid >> 6][lane] = breg[slot]; };
+  // row-major A of k-step ks: row tid/2, 8 k
+  const __bf16 *asrc = IMPLICIT ? nullptr : a.A + (rowb - a.row0 + (tid >> 1)) * a.lda + (tid & 1) * 8;
+  auto loadA = [&](int ks, int slot) {
+    if (!IMPLICIT) {
+      ks = ks < nks ? ks : nks - 1;
+      areg[slot] = *reinterpret_cast<const lac_bf16x8 *>(asrc + ks * 16);
+    }
+  };
+  auto commitA = [&](int slot, int buf) {
+    if (!IMPLICIT) *reinterpret_cast<lac_bf16x8 *>(&sA[buf][(tid >> 1) * ACOL + (tid & 1) * 8]) = areg[slot];
+  };
+  // feature-plane halo: column tid of the tile's 66 x 6 halo (threads past
+  // the halo re-read its last column into their spare LDS column)
+  const __bf16 *hsrc = nullptr;
+  if (IMPLICIT) {
+    const int t = (int)(rowb >> 8), c = tid < HALO_C ? tid : HALO_C - 1;
+    const int tj = t % R.ntj, ti = t / R.ntj;
+    hsrc = a.fp.F + ((long)(ti * 4 + c / HALO_J) * a.fp.fpj + tj * 64 + c % HALO_J) * 16;
+  }
+  const int ndg = nks / 9;
+  lac_bf16x8 hreg[2];
+  auto loadH = [&](int dg) {
+    dg = dg < ndg ? dg : ndg - 1;
+    const lac_bf16x8 *src = reinterpret_cast<const lac_bf16x8 *>(hsrc + dg * a.fp.plane());
+    hreg[0] = src[0];
+    hreg[1] = src[1];
+  };
+  auto commitH = [&](int buf) {
+    lac_bf16x8 *dst = reinterpret_cast<lac_bf16x8 *>(&sA[buf][tid * ACOL]);
+    dst[0] = hreg[0];
+    dst[1] = hreg[1];
+  };
+
+  auto mfma_step = [&](int abuf, int bbuf, int t) {
+    lac_bf16x8 av[4], bv[2];
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+      const int m = wm * 128 + 32 * x + (lane & 31);
+      if (IMPLICIT) {
+        const int cc = 2 * t + (lane >> 5), h = cc / 9, p = cc % 9;
+        const int col = ((m >> 6) + p / 3) * HALO_J + (m & 63) + p % 3;
+        av[x] = *reinterpret_cast<const lac_bf16x8 *>(&sA[abuf][col * ACOL + h * 8]);
+      } else {
+        av[x] = *reinterpret_cast<const lac_bf16x8 *>(&sA[abuf][m * ACOL + (lane >> 5) * 8]);
+      }
+    }
+#pragma unroll
+    for (int y = 0; y < 2; y++) bv[y] = sB[bbuf][wn * 2 + y][lane];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 2; y++) acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[x], bv[y], acc[x][y], 0, 0, 0);
+  };
+
+  // prologue: stage 0 into LDS; stages 1 .. PD into register slots s % PD
+  if (IMPLICIT) {
+    loadH(0);
+    commitH(0);
+  }
+  loadB(0, 0);
+  loadA(0, 0);
+  commitB(0, 0);
+  commitA(0, 0);
+#pragma unroll
+  for (int u = 1; u <= PD; u++) {
+    loadB(u, u % PD);
+    loadA(u, u % PD);
+  }
+  __syncthreads();
+  for (int k0 = 0; k0 < nks; k0 += KG) {
+    const int dg = k0 / 9;  // feature planes: the pair of fl groups of this group of k-steps
+    if (IMPLICIT) loadH(dg + 1);
+#pragma unroll
+    for (int u = 0; u < KG; u++) {
+      const int ks = k0 + u;
+      mfma_step(IMPLICIT ? (dg & 1) : (ks & 1), ks & 1, u);
+      // stage ks+1 from its register slot into the free LDS buffer, then
+      // refill that slot with stage ks+1+PD (stage s lives in slot s % PD)
+      commitB((u + 1) % PD, (ks + 1) & 1);
+      commitA((u + 1) % PD, (ks + 1) & 1);
+      if (IMPLICIT && u == KG - 1) commitH((dg + 1) & 1);
+      loadB(ks + 1 + PD, (u + 1) % PD);
+      loadA(ks + 1 + PD, (u + 1) % PD);
+      __syncthreads();
+    }
+  }
+
+  // epilogue: D[m][n] of tile (x, y): n = lane%32, m = 8*(r/4) + 4*(lane/32) + r%4
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) {
+      const int n = n0 + wn * 64 + 32 * y + (lane & 31);
+      if (n >= a.N) continue;
+      const float bn = a.bias[n];
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int m = wm * 128 + 32 * x + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const long row = rowb - a.row0 + m;
+        const float v = acc[x][y][r] + bn;
+        if (a.H)
+          a.H[row * a.ldh + n] = (__bf16)(v > 0.f ? v : 0.f);
+        else
+          a.Y[row * a.ldy + n] = v;
+      }
+    }
+}
+
 // Training targets at columns cols[r]: ordinary-substep result (the state)
 // minus the long-substep result (fu, fv, fw, fpai).
 template <typename T>
@@ -1975,6 +2198,7 @@ struct Core : CoreBase {
     T *cu = nullptr, *cv = nullptr, *cw = nullptr;  // long substep with increment
     double *cpai = nullptr;
     float *Y = nullptr;        // network outputs of every row [rtot x nout]
+    FeatPlanes fp{};           // normalised inputs of every column (fused engine)
     __bf16 *Xc = nullptr;      // features of one inference batch
     float *Xraw = nullptr, *Traw = nullptr;  // training set (ring)
     long count = 0, next = 0;
@@ -1984,7 +2208,7 @@ struct Core : CoreBase {
     float *Tn = nullptr;
     int *bidx = nullptr;
     long bcap = 0, icap = 0;
-    float *stat = nullptr;  // imean, irstd [nin]; omean, ostd [nout]
+    float *stat = nullptr;  // imean, irstd [nin]; omean, ostd [nout]; fmean, frstd [nflp]
     bool normalised = false;
     double *norms = nullptr;  // check accumulators
     int mode = MLC_LAC_OFF;
@@ -2004,8 +2228,10 @@ struct Core : CoreBase {
     if (L.exists) return mlc_fail(MLC_ESTATE, "mlc_lac_create: the handle already has a corrector");
     if (prm.nsound < 2) return mlc_fail(MLC_EINVAL, "mlc_lac_create: needs nsound >= 2");
     if (c.nhid < 16 || c.samples_per_phase < 1 || c.capacity < 1 || c.chunk_rows < 256 || c.chunk_rows % 256 ||
-        c.check_every < 0 || !(c.check_tol >= 0.0))
+        c.check_every < 0 || !(c.check_tol >= 0.0) || (c.engine != 0 && c.engine != 1))
       return mlc_fail(MLC_EINVAL, "mlc_lac_create: bad configuration");
+    if (c.engine == 1 && c.nhid % 64)
+      return mlc_fail(MLC_EINVAL, "mlc_lac_create: the fused engine needs nhid a multiple of 64");
     L.cfg = c;
     const int K = g.nk;
     L.nflp = (int)round_up(LAC_IN_FIELDS * K, 16);
@@ -2024,12 +2250,17 @@ struct Core : CoreBase {
     AL(L.ru, fT) AL(L.rv, fT) AL(L.rw, fT1) AL(L.rpai, fD);
     AL(L.cu, fT) AL(L.cv, fT) AL(L.cw, fT1) AL(L.cpai, fD);
     AL(L.Y, (size_t)L.rtot * L.nout * sizeof(float));
-    AL(L.Xc, (size_t)L.chunk * L.nin * sizeof(__bf16));
+    if (c.engine == 0) AL(L.Xc, (size_t)L.chunk * L.nin * sizeof(__bf16));
+    if (c.engine == 1) {
+      L.fp.fpj = L.rows.ntj * 64 + 2;
+      L.fp.fpi = ((g.ni + 3) / 4) * 4 + 2;
+      AL(L.fp.F, (size_t)(L.nflp / 16) * L.fp.plane() * sizeof(__bf16));
+    }
     AL(L.Xraw, (size_t)c.capacity * L.nin * sizeof(float));
     AL(L.Traw, (size_t)c.capacity * L.nout * sizeof(float));
     AL(L.cols, (size_t)c.samples_per_phase * sizeof(int));
     AL(L.Craw, (size_t)c.capacity * sizeof(int));
-    AL(L.stat, (size_t)(2 * L.nin + 2 * L.nout) * sizeof(float));
+    AL(L.stat, (size_t)(2 * L.nin + 2 * L.nout + 2 * L.nflp) * sizeof(float));
     AL(L.norms, 16 * sizeof(double));
 #undef AL
     L.hcols.assign(c.samples_per_phase, 0);
@@ -2073,18 +2304,73 @@ struct Core : CoreBase {
   int lac_infer() {
     const float *st = L.stat;
     const double nh = L.cfg.nhid;
+    int rc = mlp_reserve_rows(L.net, L.chunk);
+    if (rc) return rc;
+    if (L.cfg.engine == 1 && (rc = mlp_pack_dev(L.net))) return rc;
+    if (L.cfg.engine == 1) {
+      const float *fs = L.stat + 2 * L.nin + 2 * L.nout;
+      launch(KK_LAC_GATHER, fbytes(6, 2) + (double)(L.nflp / 16) * L.fp.plane() * sizeof(__bf16),
+             k_lac_features<T>, dim3((L.fp.fpj + 63) / 64, (L.fp.fpi + 3) / 4), dim3(256), D, lac_snap(), L.fp,
+             L.nflp / 16, fs, fs + L.nflp);
+    }
     for (long r0 = 0; r0 < L.rtot; r0 += L.chunk) {
       const int n = (int)(L.rtot - r0 < L.chunk ? L.rtot - r0 : L.chunk);
+      const double flops = 2.0 * n * (L.nin * nh + nh * nh + nh * L.nout);
+      if (L.cfg.engine == 1) {
+        timed(KK_LAC_MLP, flops, [&]() { fused_forward(r0, n); });
+        continue;
+      }
       timed(KK_LAC_GATHER, (double)n * L.nin * sizeof(__bf16) + fbytes(6, 0) * n / L.rtot, [&]() {
         hipLaunchKernelGGL(k_lac_gather<T>, dim3(n / 64), dim3(256), 0, stream, D, lac_snap(), L.rows,
                            (const int *)nullptr, r0, n, L.nin, (float *)nullptr, L.Xc, st, st + L.nin);
       });
-      int rc = MLC_OK;
-      timed(KK_LAC_MLP, 2.0 * n * (L.nin * nh + nh * nh + nh * L.nout),
-            [&]() { rc = mlp_forward_dev(L.net, L.Xc, L.nin, n, L.Y + r0 * L.nout, L.nout); });
+      timed(KK_LAC_MLP, flops, [&]() { rc = mlp_forward_dev(L.net, L.Xc, L.nin, n, L.Y + r0 * L.nout, L.nout); });
       if (rc) return rc;
     }
     return MLC_OK;
+  }
+
+  // The three layers of rows r0 .. r0+n-1 (n a multiple of 256) on the fused
+  // kernels: layer 1 gathers its inputs from the fields.
+  void fused_forward(long r0, int n) {
+    mlc_mlp *p = L.net;
+    const int nrt = n / FM;
+    auto grid = [&](int nN) { return dim3((unsigned)round_up((long)nrt * nN, 8)); };
+    FusedArgs a{};
+    a.row0 = r0;
+    a.nrt = nrt;
+    a.fp = L.fp;
+    // layer 1
+    a.Bp = reinterpret_cast<const lac_bf16x8 *>(p->Wp[0]);
+    a.bnt = p->wp_nt[0];
+    a.nks = p->wp_ks[0];
+    a.N = p->nhid;
+    a.nN = (p->nhid + FN - 1) / FN;
+    a.bias = p->bvec[0];
+    a.H = p->H1;
+    a.ldh = p->nhid;
+    hipLaunchKernelGGL(k_mlp_fused<true>, grid(a.nN), dim3(FT), 0, stream, L.rows, a);
+    // layer 2
+    a.Bp = reinterpret_cast<const lac_bf16x8 *>(p->Wp[1]);
+    a.bnt = p->wp_nt[1];
+    a.nks = p->wp_ks[1];
+    a.bias = p->bvec[1];
+    a.A = p->H1;
+    a.lda = p->nhid;
+    a.H = p->H2;
+    hipLaunchKernelGGL(k_mlp_fused<false>, grid(a.nN), dim3(FT), 0, stream, L.rows, a);
+    // layer 3
+    a.Bp = reinterpret_cast<const lac_bf16x8 *>(p->Wp[2]);
+    a.bnt = p->wp_nt[2];
+    a.nks = p->wp_ks[2];
+    a.N = p->nout;
+    a.nN = (p->nout + FN - 1) / FN;
+    a.bias = p->bvec[2];
+    a.A = p->H2;
+    a.H = nullptr;
+    a.Y = L.Y + r0 * L.nout;
+    a.ldy = L.nout;
+    hipLaunchKernelGGL(k_mlp_fused<false>, grid(a.nN), dim3(FT), 0, stream, L.rows, a);
   }
 
   void lac_apply() {
@@ -2198,14 +2484,19 @@ struct Core : CoreBase {
       fm[fl] += mean[o] / 9.0;
       fq[fl] += (sd[o] * sd[o] + mean[o] * mean[o]) / 9.0;
     }
-    std::vector<float> st(2 * nin + 2 * nout);
+    std::vector<float> st(2 * nin + 2 * nout + 2 * L.nflp);
+    float *fmean = st.data() + 2 * nin + 2 * nout, *frstd = fmean + L.nflp;
+    for (int fl = 0; fl < L.nflp; fl++) {
+      const double var = fq[fl] - fm[fl] * fm[fl];
+      const double s = var > 0.0 ? std::sqrt(var) : 0.0;
+      fmean[fl] = (float)fm[fl];
+      frstd[fl] = s > 1e-30 * (1.0 + std::fabs(fm[fl])) ? (float)(1.0 / s) : 0.f;
+    }
     for (int o = 0; o < nin; o++) {
       int fl, p;
       lac_feature(o, fl, p);
-      const double var = fq[fl] - fm[fl] * fm[fl];
-      const double s = var > 0.0 ? std::sqrt(var) : 0.0;
-      st[o] = (float)fm[fl];
-      st[nin + o] = s > 1e-30 * (1.0 + std::fabs(fm[fl])) ? (float)(1.0 / s) : 0.f;
+      st[o] = fmean[fl];
+      st[nin + o] = frstd[fl];
     }
     for (int o = 0; o < nout; o++) {
       st[2 * nin + o] = (float)mean[nin + o];
@@ -2331,7 +2622,10 @@ struct Core : CoreBase {
     }
     if (learned) {
       L.learned_steps++;
-      if (check && worst > L.cfg.check_tol) L.st.fallback = 1;
+      if (check) {
+        L.st.worst_ratio = worst;
+        if (worst > L.cfg.check_tol) L.st.fallback = 1;
+      }
     }
     if (prm.filterpai) {  // :333-341
       launch(KK_FILTER_BRACKET, fbytes(0, 3), k_sub<double, double>, pt_grid(), blk(), D.pai, (const double *)D.pf, g);
@@ -2563,6 +2857,7 @@ extern "C" int mlc_lac_default_config(mlc_lac_config *c) {
   c->check_every = 50;
   c->check_tol = 1.0;
   c->seed = 1;
+  c->engine = 1;
   return MLC_OK;
 }
 

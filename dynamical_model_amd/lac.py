@@ -140,7 +140,7 @@ class Corrector:
         check("mlc_lac_get_stats", load().mlc_lac_get_stats(self.model._h, C.byref(s)))
         return {"samples": s.samples, "learned_phases": s.learned_phases, "plain_phases": s.plain_phases,
                 "checks": s.checks, "fallback": bool(s.fallback), "last_loss": s.last_loss,
-                "ratio": list(s.ratio), "rel": list(s.rel)}
+                "ratio": list(s.ratio), "rel": list(s.rel), "worst_ratio": s.worst_ratio}
 
     def dataset(self, first: int, n: int, kz: int):
         """Rows of the training set: (X raw inputs, T targets, sampled columns)."""

@@ -59,6 +59,8 @@ typedef struct mlc_lac_config {
   int check_every;       /* check every this many MLC_LAC_ON steps (0: never)  */
   double check_tol;      /* fall back when a field's error ratio exceeds it    */
   unsigned long long seed;
+  int engine;            /* inference: 1 fused MFMA kernels reading the fields,
+                          * 0 materialised features + the generic GEMM       */
 } mlc_lac_config;
 
 typedef struct mlc_lac_stats {
@@ -73,6 +75,7 @@ typedef struct mlc_lac_stats {
    * with the increment, both against the ordinary substeps */
   double ratio[4];
   double rel[4];
+  double worst_ratio;       /* largest ratio over the phases of the last check */
 } mlc_lac_stats;
 
 int mlc_lac_default_config(mlc_lac_config *c);

@@ -178,6 +178,37 @@ def test_learned_increment_beats_long_substep(precision):
         m.close()
 
 
+@pytest.mark.parametrize("shape", [(64, 64, 16), (70, 45, 20)])
+def test_fused_engine_matches_generic(shape):
+    """The fused kernels (inputs gathered from the fields inside the MFMA
+    kernel, packed weights) against materialised features + the generic GEMM:
+    same trained network, one learned step each."""
+    jx, iy, kz = shape
+    case = small_config2(jx, iy, kz)
+    runs = {}
+    for engine in (0, 1):
+        m, _, _ = _model(case, precision=dm.MLC_FP32)
+        corr = Corrector(m, nhid=128, samples_per_phase=1024, capacity=8192, chunk_rows=1024, check_every=0,
+                         seed=5, engine=engine)
+        print(f"case {jx}x{iy}x{kz} fp32 engine={engine}", flush=True)
+        corr.collect(2)
+        corr.train(200, 256, 1e-3)
+        w = [corr.network(kz).get_weights(l) for l in range(3)]
+        corr.mode("on")
+        m.moloch(3)
+        runs[engine] = (w, _fields(m), corr.stats())
+        m.close()
+    for l in range(3):
+        assert np.array_equal(runs[0][0][l][0], runs[1][0][l][0])
+    assert runs[1][2]["learned_phases"] == 3
+    for n in PROGNOSTIC:
+        a, b = dm.interior(runs[1][1][n]), dm.interior(runs[0][1][n])
+        rel = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        print(f"{n:6s} fused vs generic rel {rel:.2e}")
+        # both engines issue the same MFMA sequence in the same k order
+        assert np.array_equal(a, b), n
+
+
 def test_failed_check_falls_back_bitwise():
     case = small_config2(48, 40, 16)
     a, _, _ = _model(case, precision=dm.MLC_FP32)
