@@ -89,20 +89,35 @@ def run_variant(dm, hs, case, precision: int, steps: int, warmup: int, lead: int
 
 
 MFMA_KINDS = {"lac_mlp"}  # work counted in flops; every other kind in bytes
+
+
+def traffic_table(jx: int, iy: int, kz: int, precision: str) -> dict:
+    """Measured HBM bytes per launch of each kernel kind (rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs, calibrated on device copies of known size), if a table for
+    this workload is committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{jx}x{iy}x{kz}_{precision}.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f).get("kinds", {})
 MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense BF16
 
 
-def roofline_of(prof: dict, name: str) -> dict:
+def roofline_of(prof: dict, name: str, traffic: dict) -> dict:
     ms, n, work = prof[name]
+    measured = traffic.get(name, {}).get("bytes")
     if name in MFMA_KINDS:
         achieved = work / (ms * 1e-3) / 1e12
         return {"bound": "mfma", "achieved": achieved, "peak": MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / MFMA_PEAK_TFLOPS, "traffic": None, "kernel": name,
+                "frac": achieved / MFMA_PEAK_TFLOPS, "traffic": measured, "kernel": name,
                 "flops_per_launch": work / n, "avg_launch_ms": ms / n}
     achieved = work / (ms * 1e-3) / 1e9
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name,
-            "bytes_per_launch": work / n, "avg_launch_ms": ms / n}
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": measured, "kernel": name,
+           "bytes_per_launch": work / n, "avg_launch_ms": ms / n}
+    if measured:
+        out["measured_gbs"] = measured / (ms / n * 1e-3) / 1e9
+    return out
 
 
 def dominant(prof: dict) -> tuple[str, float, float, int]:
@@ -141,7 +156,7 @@ def main() -> int:
     ap.add_argument("--jx", type=int, default=0)
     ap.add_argument("--iy", type=int, default=0)
     ap.add_argument("--kz", type=int, default=0)
-    ap.add_argument("--cpu-sample", default="128x128x4", help="JXxIYxSTEPS of the CPU oracle sample, or 'none'")
+    ap.add_argument("--cpu-sample", default="128x128x10", help="JXxIYxSTEPS of the CPU oracle sample, or 'none'")
     ap.add_argument("--lead", type=int, default=6,
                     help="ordinary steps before warm-up (the corrector's training run)")
     ap.add_argument("--train-iters", type=int, default=1600, help="Adam steps of the corrector")
@@ -203,9 +218,11 @@ def main() -> int:
 
     chosen_tag = "learned" if "learned" in results else ("fp32" if "fp32" in results else "fp64")
     chosen = results[chosen_tag]
-    roofline = roofline_of(chosen["_prof"], dominant(chosen["_prof"])[0])
+    traffic = traffic_table(jx, iy, kz, "fp64" if chosen_tag == "fp64" else "fp32")
+    roofline = roofline_of(chosen["_prof"], dominant(chosen["_prof"])[0], traffic)
     stencil = roofline_of(chosen["_prof"], max((k for k in chosen["_prof"] if k not in MFMA_KINDS),
-                                               key=lambda k: chosen["_prof"][k][0]))
+                                               key=lambda k: chosen["_prof"][k][0]), traffic)
+    mlp = roofline_of(chosen["_prof"], "lac_mlp", traffic) if "lac_mlp" in chosen["_prof"] else None
 
     cpu = None
     if args.cpu_sample != "none":
@@ -247,6 +264,7 @@ def main() -> int:
         "rel_l2_after_steps": args.lead + args.warmup + args.steps + 1,
         "roofline": roofline,
         "stencil_roofline": stencil,
+        "mlp_roofline": mlp,
         "cpu_baseline": cpu,
         "variants": results,
     }

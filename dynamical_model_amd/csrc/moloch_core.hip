@@ -852,7 +852,7 @@ __device__ __forceinline__ long xcd_remap(long b, long total) {
 // result.  Everything that does not depend on the advected field (Courant
 // numbers, fmz ratios, the divergence-correction coefficients) is evaluated
 // once per block; each face flux once per field.
-template <typename T, typename Q, int NF>
+template <typename T, typename Q, int NF, bool ROT>
 __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, double dta, int ntj, int nti) {
   __shared__ Q sWz[HEI][HEJ];       // wz, rows i0-2 .. i0+HI+1
   __shared__ Q sPp[2][HI][HEJ];     // field, rows i0 .. i0+HI-1 (double buffered)
@@ -883,7 +883,7 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, do
     if (f <= HI) {
       const int iw = wi(i0 + f);
       Q vv = (Q)D.v[g.at(jw, iw, k)];
-      if (P.rotllr) {
+      if constexpr (ROT) {
         mz[n] = vv * dtrdy;
         ma[n] = Q(0.5) * vv;
       } else {
@@ -901,7 +901,7 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, do
     const int iw = wi(i0 + r), ip = wi(i0 + r + 1), im = wi(i0 + r - 1);
     Q f0 = (Q)D.fmz[g.at(jw, iw, k)], fn = (Q)D.fmz[g.at(jw, ip, k)], fs = (Q)D.fmz[g.at(jw, im, k)];
     Q vc = (Q)D.v[g.at(jw, iw, k)], vp = (Q)D.v[g.at(jw, ip, k)];
-    if (P.rotllr) {
+    if constexpr (ROT) {
       Q mx = (Q)D.mx[g.at2(jw, iw)];
       Q zhxvtn = dtrdy * (Q)D.rmv[g.at2(jw, ip)] * mx;
       Q zhxvts = dtrdy * (Q)D.rmv[g.at2(jw, iw)] * mx;
@@ -927,7 +927,7 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, do
       const int jf = q % (HJ + 1), r = q / (HJ + 1);
       const int jw2 = wj(j0 + jf), iw = wi(i0 + r);
       Q uu = (Q)D.u[g.at(jw2, iw, k)];
-      if (P.rotllr) {
+      if constexpr (ROT) {
         zz[n] = uu * (Q)D.mu[g.at2(jw2, iw)] * dtrdx;
         za[n] = Q(0.5) * uu;
       } else {
@@ -948,7 +948,7 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, do
       const int j = wj(j0 + jt), jp = wj(j0 + jt + 1), jm = wj(j0 + jt - 1), iw = wi(i0 + r);
       Q f0 = (Q)D.fmz[g.at(j, iw, k)], fe = (Q)D.fmz[g.at(jp, iw, k)], fw = (Q)D.fmz[g.at(jm, iw, k)];
       Q uc = (Q)D.u[g.at(j, iw, k)], up = (Q)D.u[g.at(jp, iw, k)];
-      if (P.rotllr) {
+      if constexpr (ROT) {
         Q zcostx = dtrdx * (Q)D.mu[g.at2(j, iw)];
         zfe[n] = zcostx * Q(2) * f0 / (f0 + fe);
         zfw[n] = zcostx * Q(2) * f0 / (f0 + fw);
@@ -1012,7 +1012,7 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, do
       const int r = rr + 4 * n;
       Q zdv = pcd[n] * sPp[pb][r][ej];
       Q o;
-      if (P.rotllr)
+      if constexpr (ROT)
         o = sWz[r + 2][ej] + sZp[r][ej] * pfs[n] - sZp[r + 1][ej] * pfn[n] + zdv;
       else
         o = sWz[r + 2][ej] + pm2[n] * (sZp[r][ej] * pfs[n] - sZp[r + 1][ej] * pfn[n] + zdv);
@@ -1056,7 +1056,7 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, do
         if (j <= nj && i <= ni) {
           Q zdv = zcd[n] * sPp[pb][r][jt + 2];
           Q o;
-          if (P.rotllr)
+          if constexpr (ROT)
             o = sP0[r][jt + 2] + sZw[r][jt] * zfw[n] - sZw[r][jt + 1] * zfe[n] + zdv;
           else
             o = sP0[r][jt + 2] + zm2[n] * (sZw[r][jt] * zfw[n] - sZw[r][jt + 1] * zfe[n] + zdv);
@@ -2123,8 +2123,12 @@ struct Core : CoreBase {
     // horizontal: u, v, fmz + NF x (wz, field) -> NF results
     const int ntj = (g.nj + HJ - 1) / HJ, nti = (g.ni + HI - 1) / HI;
     const long total = (long)ntj * nti * g.nk;
-    launch(KK_WAF_H, fbytes(3, 0) + (q ? fbytes(0, 3 * NF) : fbytes(3 * NF, 0)), k_wafh<T, Q, NF>,
-           dim3((unsigned)round_up(total, 8)), dim3(256), D, P, F, dta, ntj, nti);
+    if (prm.rotllr)
+      launch(KK_WAF_H, fbytes(3, 0) + (q ? fbytes(0, 3 * NF) : fbytes(3 * NF, 0)), k_wafh<T, Q, NF, true>,
+             dim3((unsigned)round_up(total, 8)), dim3(256), D, P, F, dta, ntj, nti);
+    else
+      launch(KK_WAF_H, fbytes(3, 0) + (q ? fbytes(0, 3 * NF) : fbytes(3 * NF, 0)), k_wafh<T, Q, NF, false>,
+             dim3((unsigned)round_up(total, 8)), dim3(256), D, P, F, dta, ntj, nti);
     (void)sT;
     for (int f = 0; f < NF; f++) std::swap(*fld[f], outs[f]);
   }

@@ -262,11 +262,11 @@ def test_corrector_errors():
     case = small_config2(32, 32, 8)
     a, _, _ = _model(case, nsound=1)
     with pytest.raises(dm.MolochError) as e:
-        Corrector(a, nhid=32, samples_per_phase=16, capacity=64, chunk_rows=256)
+        Corrector(a, nhid=64, samples_per_phase=16, capacity=64, chunk_rows=256)
     assert e.value.code == _lib.MLC_EINVAL
     a.close()
     b, _, _ = _model(case)
-    corr = Corrector(b, nhid=32, samples_per_phase=16, capacity=64, chunk_rows=256)
+    corr = Corrector(b, nhid=64, samples_per_phase=16, capacity=64, chunk_rows=256)
     with pytest.raises(dm.MolochError) as e:
         corr.mode("on")  # not trained
     assert e.value.code == _lib.MLC_ESTATE
@@ -274,5 +274,5 @@ def test_corrector_errors():
         corr.train(1, 8, 1e-3)  # nothing collected
     assert e.value.code == _lib.MLC_ESTATE
     with pytest.raises(dm.MolochError):
-        Corrector(b, nhid=32, samples_per_phase=16, capacity=64, chunk_rows=256)  # second corrector
+        Corrector(b, nhid=64, samples_per_phase=16, capacity=64, chunk_rows=256)  # second corrector
     b.close()
