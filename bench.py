@@ -120,6 +120,15 @@ def roofline_of(prof: dict, name: str, traffic: dict) -> dict:
     return out
 
 
+def step_bytes(kz: int, fp64: bool) -> float:
+    """Algorithmic bytes of one column-step of the ordinary step: 470 field
+    passes of kz levels (prologue 11, 15 acoustic substeps x 25, 3 advection
+    cycles x 24, epilogue 12); 53 of them are Exner-function passes, which stay
+    FP64 in the FP32 core."""
+    s = 8 if fp64 else 4
+    return (470 * s + 53 * (8 - s)) * kz
+
+
 def dominant(prof: dict) -> tuple[str, float, float, int]:
     name = max(prof, key=lambda k: prof[k][0])
     ms, n, b = prof[name]
@@ -208,6 +217,10 @@ def main() -> int:
         }
         if extra:
             results[tag]["corrector"] = extra
+        else:
+            sb = step_bytes(kz, tag == "fp64") * jx * iy
+            results[tag]["step_roofline"] = {"bytes_per_step": sb, "achieved_gbs": sb / (secs / args.steps) / 1e9,
+                                             "frac": sb / (secs / args.steps) / 1e9 / HBM_PEAK_GBS}
         finals[tag] = fin
         log(f"{tag}: {cs:.4g} column-steps/s, {1e3 * secs / args.steps:.2f} ms/step, dominant {name} "
             f"({time.perf_counter() - t0:.1f} s incl. set-up)")

@@ -1715,7 +1715,16 @@ struct Core : CoreBase {
   double acc_bytes[KK_COUNT] = {0};
 
   // Algorithmic bytes of one launch: nT passes over kz-level fields of the
-  // storage type, nD passes over FP64 fields (the Exner function).
+  // storage type, nD passes over FP64 fields (the Exner function).  A pass is
+  // a field that must cross a grid-wide synchronisation of the step (read or
+  // written once per phase); each kernel is charged the passes of its share
+  // of the step: prologue 11, per acoustic substep 25 (sound_a: u, v, fmz;
+  // sound_b: the other 15 of the divergence / column solve / theta-pi phase;
+  // sound_c: the 7 of the momentum phase), per advection cycle the s write
+  // and the u, v, w reads (adv_in), the advected fields read once with s,
+  // fmz, fmzf (waf_v) and written once (waf_h), the u, v, w writes
+  // (adv_out), epilogue 12.  Intermediate arrays of this implementation
+  // (zdiv2, the elimination coefficients, wz, p0) are not charged.
   double fbytes(double nT, double nD) const {
     return (double)g.nj * g.ni * g.nk * (nT * sizeof(T) + nD * sizeof(double));
   }
@@ -2066,9 +2075,9 @@ struct Core : CoreBase {
     Par P = par();
     halo(D.u, g.nk, 1);
     halo(D.v, g.nk, 1);
-    launch(KK_SOUND_A, fbytes(5, 0), k_sound_a<T>, col_grid(), blk(), D, P, dts);
+    launch(KK_SOUND_A, fbytes(3, 0), k_sound_a<T>, col_grid(), blk(), D, P, dts);
     int split = prm.divfilter ? 1 : 0;
-    launch(KK_SOUND_B, fbytes(18, 2), k_sound_b<T>, col_grid(), blk(), D, P, dts, ns, split);
+    launch(KK_SOUND_B, fbytes(13, 2), k_sound_b<T>, col_grid(), blk(), D, P, dts, ns, split);
     if (split) {
       // divergence_filter (:423-438) then the Exner update
       halo(D.zds, g.nk, 0);
@@ -2096,11 +2105,11 @@ struct Core : CoreBase {
     Q *wz = (Q *)D.wz, *p0 = (Q *)D.p0;
     // vertical: pp, s, fmz, fmzf -> wz;  meridional: wz, v, fmz, pp -> p0;
     // zonal: p0, u, fmz, pp -> pp
-    launch(KK_WAF_V, fbytes(3 + 2 * (1 - q), 2 * q), k_waf_v<T, Q>, col_grid(), blk(), D, P, (const Q *)pp, wz, dta,
+    launch(KK_WAF_V, fbytes(3 + (1 - q), q), k_waf_v<T, Q>, col_grid(), blk(), D, P, (const Q *)pp, wz, dta,
            prm.vadvtwice ? 1 : 0);
-    launch(KK_WAF_M, fbytes(2 + 3 * (1 - q), 3 * q), k_waf_m<T, Q>, pt_grid(), blk(), D, P, (const Q *)pp,
+    launch(KK_WAF_M, 0.0, k_waf_m<T, Q>, pt_grid(), blk(), D, P, (const Q *)pp,
            (const Q *)wz, p0, dta);
-    launch(KK_WAF_Z, fbytes(2 + 3 * (1 - q), 3 * q), k_waf_z<T, Q>, pt_grid(), blk(), D, P, pp, (const Q *)p0, dta);
+    launch(KK_WAF_Z, fbytes(1 - q, q), k_waf_z<T, Q>, pt_grid(), blk(), D, P, pp, (const Q *)p0, dta);
   }
 
   // wafone on a group of fields of one storage type: k_wafv then k_wafh; each
@@ -2118,16 +2127,16 @@ struct Core : CoreBase {
     const double q = sizeof(Q) == sizeof(double) ? 1.0 : 0.0;
     const double sT = sizeof(T) == sizeof(double) ? 1.0 : 0.0;
     // vertical: s, fmz, fmzf + NF fields -> NF wz
-    launch(KK_WAF_V, fbytes(3, 0) + (q ? fbytes(0, 2 * NF) : fbytes(2 * NF, 0)), k_wafv<T, Q, NF>, col_grid(),
+    launch(KK_WAF_V, fbytes(3, 0) + (q ? fbytes(0, NF) : fbytes(NF, 0)), k_wafv<T, Q, NF>, col_grid(),
            blk(), D, P, F, dta);
     // horizontal: u, v, fmz + NF x (wz, field) -> NF results
     const int ntj = (g.nj + HJ - 1) / HJ, nti = (g.ni + HI - 1) / HI;
     const long total = (long)ntj * nti * g.nk;
     if (prm.rotllr)
-      launch(KK_WAF_H, fbytes(3, 0) + (q ? fbytes(0, 3 * NF) : fbytes(3 * NF, 0)), k_wafh<T, Q, NF, true>,
+      launch(KK_WAF_H, q ? fbytes(0, NF) : fbytes(NF, 0), k_wafh<T, Q, NF, true>,
              dim3((unsigned)round_up(total, 8)), dim3(256), D, P, F, dta, ntj, nti);
     else
-      launch(KK_WAF_H, fbytes(3, 0) + (q ? fbytes(0, 3 * NF) : fbytes(3 * NF, 0)), k_wafh<T, Q, NF, false>,
+      launch(KK_WAF_H, q ? fbytes(0, NF) : fbytes(NF, 0), k_wafh<T, Q, NF, false>,
              dim3((unsigned)round_up(total, 8)), dim3(256), D, P, F, dta, ntj, nti);
     (void)sT;
     for (int f = 0; f < NF; f++) std::swap(*fld[f], outs[f]);
@@ -2150,7 +2159,7 @@ struct Core : CoreBase {
   void advection(double dta) {
     halo(D.u, g.nk, 2);
     halo(D.v, g.nk, 3);
-    launch(KK_ADV_IN, fbytes(9, 0), k_adv_in<T>, col_grid(), blk(), D);
+    launch(KK_ADV_IN, fbytes(4, 0), k_adv_in<T>, col_grid(), blk(), D);
     if (prm.vadvtwice && !unfused_advection) {
       T **qm[5] = {&D.qv, &D.qc, &D.qi, &D.qr, &D.qs};
       T **list[10];
@@ -2171,7 +2180,7 @@ struct Core : CoreBase {
       }
       halo(D.ux, g.nk, 2);
       halo(D.vx, g.nk, 3);
-      launch(KK_ADV_OUT, fbytes(6, 0), k_adv_out<T>, col_grid(), blk(), D);
+      launch(KK_ADV_OUT, fbytes(3, 0), k_adv_out<T>, col_grid(), blk(), D);
       return;
     }
     wafone<T>(D.tetav, dta);
@@ -2184,7 +2193,7 @@ struct Core : CoreBase {
     for (int n = prm.iqfrst; n <= dims.nqx; n++) wafone<T>(q[n - 1], dta);
     halo(D.ux, g.nk, 2);
     halo(D.vx, g.nk, 3);
-    launch(KK_ADV_OUT, fbytes(6, 0), k_adv_out<T>, col_grid(), blk(), D);
+    launch(KK_ADV_OUT, fbytes(3, 0), k_adv_out<T>, col_grid(), blk(), D);
   }
 
   // ------------------------------------------- learned acoustic corrector
