@@ -166,9 +166,11 @@ def main() -> int:
     ap.add_argument("--iy", type=int, default=0)
     ap.add_argument("--kz", type=int, default=0)
     ap.add_argument("--cpu-sample", default="128x128x10", help="JXxIYxSTEPS of the CPU oracle sample, or 'none'")
-    ap.add_argument("--lead", type=int, default=6,
+    ap.add_argument("--lead", type=int, default=10,
                     help="ordinary steps before warm-up (the corrector's training run)")
-    ap.add_argument("--train-iters", type=int, default=1600, help="Adam steps of the corrector")
+    ap.add_argument("--train-iters", type=int, default=6000, help="Adam steps of the corrector")
+    ap.add_argument("--train-batch", type=int, default=1024, help="rows per Adam step")
+    ap.add_argument("--samples", type=int, default=8192, help="training columns sampled per acoustic phase")
     ap.add_argument("--variants", default="fp64,fp32,learned")
     ap.add_argument("--check-tol", type=float, default=1.0,
                     help="corrector check: fall back when a field's error ratio exceeds this")
@@ -189,9 +191,10 @@ def main() -> int:
     log(f"built configuration {args.config} ({jx}x{iy}x{kz}) on the host in {time.perf_counter() - t0:.1f} s")
 
     learned_cfg = {
-        "config": dict(nhid=512, samples_per_phase=4096, capacity=131072, chunk_rows=65536, check_every=50,
+        "config": dict(nhid=512, samples_per_phase=args.samples,
+                       capacity=max(131072, 3 * args.lead * args.samples), chunk_rows=65536, check_every=50,
                        check_tol=args.check_tol, seed=11, engine=1),
-        "batch": 512,
+        "batch": args.train_batch,
         "schedule": [(args.train_iters // 2, 1e-3), (args.train_iters // 4, 3e-4), (args.train_iters // 4, 1e-4)],
     }
     variants = {"fp64": (dm.MLC_FP64, None), "fp32": (dm.MLC_FP32, None), "learned": (dm.MLC_FP32, learned_cfg)}
