@@ -2511,10 +2511,18 @@ struct Core : CoreBase {
       st[o] = fmean[fl];
       st[nin + o] = frstd[fl];
     }
-    for (int o = 0; o < nout; o++) {
-      st[2 * nin + o] = (float)mean[nin + o];
-      const double s = sd[nin + o];
-      st[2 * nin + nout + o] = s > 0.0 ? (float)s : 1.f;
+    // targets: mean per output, spread per field (root mean square of the
+    // levels' standard deviations), so that the loss weights levels by their
+    // size as the fields' L2 errors do
+    for (int f = 0; f < LAC_OUT_FIELDS; f++) {
+      double q = 0.0;
+      for (int k = 0; k < g.nk; k++) q += sd[nin + f * g.nk + k] * sd[nin + f * g.nk + k];
+      const double s = std::sqrt(q / g.nk);
+      for (int k = 0; k < g.nk; k++) {
+        const int o = f * g.nk + k;
+        st[2 * nin + o] = (float)mean[nin + o];
+        st[2 * nin + nout + o] = s > 0.0 ? (float)s : 1.f;
+      }
     }
     HIP_TRY(hipMemcpyAsync(L.stat, st.data(), st.size() * sizeof(float), hipMemcpyHostToDevice, stream));
     HIP_TRY(hipStreamSynchronize(stream));
