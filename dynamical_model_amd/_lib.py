@@ -131,6 +131,14 @@ EXPORTS = {
     "mlc_lac_get_stats": (C.c_int, [C.c_void_p, C.POINTER(LacStats)]),
     "mlc_lac_network": (C.c_void_p, [C.c_void_p]),
     "mlc_lac_dataset": (C.c_int, [C.c_void_p, C.c_long, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "mlc_nc_create": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "mlc_nc_append": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mlc_nc_records": (C.c_int, [C.c_void_p]),
+    "mlc_nc_close": (C.c_int, [C.c_void_p]),
+    "mlc_output_open": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    "mlc_output_write": (C.c_int, [C.c_void_p]),
+    "mlc_output_flush": (C.c_int, [C.c_void_p]),
+    "mlc_output_close": (C.c_int, [C.c_void_p]),
     "mlc_profile": (C.c_int, [C.c_void_p, C.c_int]),
     "mlc_profile_kinds": (C.c_int, []),
     "mlc_profile_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_double),

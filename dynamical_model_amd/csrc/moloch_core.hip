@@ -27,14 +27,19 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <new>
+#include <thread>
 #include <string>
 #include <vector>
 
 #include "mlc_constants.h"
 #include "mlc_internal.h"
 #include "moloch_lac.h"
+#include "moloch_output.h"
 
 // ------------------------------------------------------------------ errors
 static thread_local char g_errbuf[512] = "no error";
@@ -1652,6 +1657,18 @@ __global__ void __launch_bounds__(256) k_diag_final(double *partial, long nblk, 
     }
 }
 
+// ============================================================ output packing
+// Interior points of a field, j fastest, as doubles: the layout of the
+// reference's output records (nompi/mod_mppparam.F90:197,310).
+template <typename F>
+__global__ void __launch_bounds__(256) k_pack_interior(const F *src, Geo g, double *dst) {
+  const int j = blockIdx.x * 64 + threadIdx.x % 64 + 1;
+  const int i = blockIdx.y * 4 + threadIdx.x / 64 + 1;
+  const int k = blockIdx.z + 1;
+  if (j > g.nj || i > g.ni) return;
+  dst[((long)(k - 1) * g.ni + (i - 1)) * g.nj + (j - 1)] = (double)src[g.at(j, i, k)];
+}
+
 // ============================================================ host driver
 // Kernel kinds for the per-launch event timing of mlc_profile_*.
 enum KernelKind {
@@ -1680,6 +1697,10 @@ struct CoreBase {
   virtual int lac_stats(mlc_lac_stats *s) = 0;
   virtual mlc_mlp *lac_network() = 0;
   virtual int lac_dataset(long first, int n, float *X, float *T, int *cols) = 0;
+  // ps then t (interior, doubles) into dst on the handle's stream; done is
+  // recorded after them; wait_event makes the stream wait for an event
+  virtual int pack_output(double *dst, hipEvent_t done) = 0;
+  virtual int wait_event(hipEvent_t e) = 0;
 };
 
 static long round_up(long a, long b) { return (a + b - 1) / b * b; }
@@ -2597,6 +2618,23 @@ struct Core : CoreBase {
 
   mlc_mlp *lac_network() override { return L.exists ? L.net : nullptr; }
 
+  int pack_output(double *dst, hipEvent_t done) override {
+    if (!initialised) return mlc_fail(MLC_ESTATE, "mlc_output_write before mlc_init");
+    const long n2 = (long)g.nj * g.ni;
+    hipLaunchKernelGGL(k_pack_interior<T>, dim3((g.nj + 63) / 64, (g.ni + 3) / 4, 1), dim3(256), 0, stream,
+                       (const T *)D.ps, g, dst);
+    hipLaunchKernelGGL(k_pack_interior<T>, dim3((g.nj + 63) / 64, (g.ni + 3) / 4, g.nk), dim3(256), 0, stream,
+                       (const T *)D.t, g, dst + n2);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(done, stream));
+    return MLC_OK;
+  }
+
+  int wait_event(hipEvent_t e) override {
+    HIP_TRY(hipStreamWaitEvent(stream, e, 0));
+    return MLC_OK;
+  }
+
   int lac_dataset(long first, int n, float *X, float *Tt, int *cols) override {
     if (!L.exists) return mlc_fail(MLC_ESTATE, "mlc_lac_dataset: no corrector (mlc_lac_create)");
     if (first < 0 || n < 0 || first + n > L.count) return mlc_fail(MLC_EINVAL, "mlc_lac_dataset: rows out of range");
@@ -2912,4 +2950,145 @@ extern "C" mlc_mlp *mlc_lac_network(mlc_handle *h) { return h ? h->core->lac_net
 extern "C" int mlc_lac_dataset(mlc_handle *h, long first, int n, float *X, float *T, int *cols) {
   if (!h) return mlc_fail(MLC_EINVAL, "mlc_lac_dataset: NULL handle");
   return h->core->lac_dataset(first, n, X, T, cols);
+}
+
+// ================================================================= output
+struct mlc_output {
+  mlc_handle *h = nullptr;
+  mlc_ncfile *fps = nullptr, *ft = nullptr;
+  int device = 0;
+  long n2 = 0, n3 = 0;  // values of ps and of t per record
+  hipStream_t ost = nullptr;
+  hipEvent_t ready[2] = {}, copied[2] = {};
+  double *dstage[2] = {}, *hbuf[2] = {};
+  bool busy[2] = {false, false};
+  int next = 0;
+  std::deque<int> queue;
+  std::mutex m;
+  std::condition_variable cv;
+  bool stop = false;
+  std::string err;
+  std::thread writer;
+
+  void run() {
+    (void)hipSetDevice(device);
+    for (;;) {
+      int s;
+      {
+        std::unique_lock<std::mutex> lk(m);
+        cv.wait(lk, [&] { return stop || !queue.empty(); });
+        if (queue.empty()) return;
+        s = queue.front();
+        queue.pop_front();
+      }
+      std::string e;
+      if (hipEventSynchronize(copied[s]) != hipSuccess)
+        e = "device-to-host copy of an output record failed";
+      else if (mlc_nc_append(fps, hbuf[s]) != MLC_OK || mlc_nc_append(ft, hbuf[s] + n2) != MLC_OK)
+        e = mlc_last_error();
+      std::lock_guard<std::mutex> lk(m);
+      if (!e.empty() && err.empty()) err = e;
+      busy[s] = false;
+      cv.notify_all();
+    }
+  }
+};
+
+static void output_release(mlc_output *o) {
+  if (o->writer.joinable()) {
+    {
+      std::lock_guard<std::mutex> lk(o->m);
+      o->stop = true;
+    }
+    o->cv.notify_all();
+    o->writer.join();
+  }
+  for (int s = 0; s < 2; s++) {
+    if (o->hbuf[s]) (void)hipHostFree(o->hbuf[s]);
+    if (o->dstage[s]) (void)hipFree(o->dstage[s]);
+    if (o->ready[s]) (void)hipEventDestroy(o->ready[s]);
+    if (o->copied[s]) (void)hipEventDestroy(o->copied[s]);
+  }
+  if (o->ost) (void)hipStreamDestroy(o->ost);
+  (void)mlc_nc_close(o->fps);
+  (void)mlc_nc_close(o->ft);
+  delete o;
+}
+
+extern "C" int mlc_output_open(mlc_handle *h, const char *dir, mlc_output **out) {
+  if (!h || !dir || !out) return mlc_fail(MLC_EINVAL, "mlc_output_open: NULL argument");
+  *out = nullptr;
+  auto *o = new (std::nothrow) mlc_output();
+  if (!o) return mlc_fail(MLC_ENOMEM, "mlc_output_open: out of host memory");
+  o->h = h;
+  o->device = h->prm.device;
+  o->n2 = (long)h->dims.jx * h->dims.iy;
+  o->n3 = o->n2 * h->dims.kz;
+  const std::string d(dir);
+  int rc = mlc_nc_create((d + "/ps.nc").c_str(), "ps", h->dims.jx, h->dims.iy, 0, &o->fps);
+  if (!rc) rc = mlc_nc_create((d + "/t.nc").c_str(), "t", h->dims.jx, h->dims.iy, h->dims.kz, &o->ft);
+  const size_t bytes = (size_t)(o->n2 + o->n3) * sizeof(double);
+  auto hip_ok = [&](hipError_t e, const char *what) {
+    if (e != hipSuccess && !rc) rc = mlc_fail(MLC_EHIP, "mlc_output_open: %s: %s", what, hipGetErrorString(e));
+  };
+  if (!rc) hip_ok(hipSetDevice(o->device), "hipSetDevice");
+  if (!rc) hip_ok(hipStreamCreateWithFlags(&o->ost, hipStreamNonBlocking), "stream");
+  for (int s = 0; s < 2 && !rc; s++) {
+    hip_ok(hipEventCreateWithFlags(&o->ready[s], hipEventDisableTiming), "event");
+    if (!rc) hip_ok(hipEventCreateWithFlags(&o->copied[s], hipEventDisableTiming), "event");
+    if (!rc) hip_ok(hipMalloc((void **)&o->dstage[s], bytes), "device staging");
+    if (!rc) hip_ok(hipHostMalloc((void **)&o->hbuf[s], bytes, hipHostMallocDefault), "pinned host buffer");
+  }
+  if (rc) {
+    output_release(o);
+    return rc;
+  }
+  o->writer = std::thread([o] { o->run(); });
+  *out = o;
+  return MLC_OK;
+}
+
+extern "C" int mlc_output_write(mlc_output *o) {
+  if (!o) return mlc_fail(MLC_EINVAL, "mlc_output_write: NULL");
+  const int s = o->next;
+  {
+    std::unique_lock<std::mutex> lk(o->m);
+    o->cv.wait(lk, [&] { return !o->busy[s] || !o->err.empty(); });
+    if (!o->err.empty()) return mlc_fail(MLC_EINVAL, "mlc_output_write: %s", o->err.c_str());
+    o->busy[s] = true;
+  }
+  o->next ^= 1;
+  // the slot's previous copy has finished (its file write has), so packing
+  // over its staging buffer on the handle's stream is safe
+  int rc = o->h->core->pack_output(o->dstage[s], o->ready[s]);
+  hipError_t e = hipSuccess;
+  if (!rc) e = hipStreamWaitEvent(o->ost, o->ready[s], 0);
+  if (!rc && e == hipSuccess)
+    e = hipMemcpyAsync(o->hbuf[s], o->dstage[s], (size_t)(o->n2 + o->n3) * sizeof(double), hipMemcpyDeviceToHost,
+                       o->ost);
+  if (!rc && e == hipSuccess) e = hipEventRecord(o->copied[s], o->ost);
+  if (!rc && e != hipSuccess) rc = mlc_fail(MLC_EHIP, "mlc_output_write: %s", hipGetErrorString(e));
+  std::lock_guard<std::mutex> lk(o->m);
+  if (rc) {
+    o->busy[s] = false;
+    return rc;
+  }
+  o->queue.push_back(s);
+  o->cv.notify_all();
+  return MLC_OK;
+}
+
+extern "C" int mlc_output_flush(mlc_output *o) {
+  if (!o) return mlc_fail(MLC_EINVAL, "mlc_output_flush: NULL");
+  std::unique_lock<std::mutex> lk(o->m);
+  o->cv.wait(lk, [&] { return (!o->busy[0] && !o->busy[1]) || !o->err.empty(); });
+  if (!o->err.empty()) return mlc_fail(MLC_EINVAL, "mlc_output: %s", o->err.c_str());
+  return MLC_OK;
+}
+
+extern "C" int mlc_output_close(mlc_output *o) {
+  if (!o) return MLC_OK;
+  int rc = mlc_output_flush(o);
+  output_release(o);
+  return rc;
 }

@@ -22,6 +22,7 @@ program dropin_driver
   character(len=256) :: arg , outfile
   real(rkx) :: hill , hw , rr , dl
   integer(ik4) :: i , j , k , u , nsteps , prec
+  integer(ik4) , parameter :: iout_drv = 2
   logical :: lprint
 
   call get_command_argument(1,arg) ; read(arg,*) jx
@@ -138,10 +139,20 @@ program dropin_driver
   call fill_atmosphere
   call init_moloch
 
+  ! output(istep) before moloch(istep) (nompi/regcm_dynamical_core.F90:48-52,
+  ! :202-215): ps.nc and t.nc beside the state file, every iout_drv steps
+  k = index(outfile,'/',back=.true.)
+  if ( k > 0 ) then
+    call moloch_output_open(outfile(1:k-1))
+  else
+    call moloch_output_open('.')
+  end if
   do i = 1 , nsteps
+    if ( i == 1 .or. mod(i,iout_drv) == 0 ) call moloch_output_write
     lprint = mod(i,iprint) == 0
     call moloch(i,lprint)
   end do
+  call moloch_output_close
 
   open(newunit=u,file=trim(outfile),access='stream',form='unformatted', &
        status='replace')

@@ -33,6 +33,9 @@ module mod_moloch
 
   public :: allocate_moloch , init_moloch , moloch
   public :: moloch_download , moloch_finalize
+  ! ps.nc / t.nc written from the device, in place of the driver's
+  ! grid_nc_create / grid_nc_write / grid_nc_destroy of sfs%ps and mo_atm%t
+  public :: moloch_output_open , moloch_output_write , moloch_output_close
   ! run options, set before init_moloch
   public :: moloch_precision , moloch_halo_mpi , moloch_resident , moloch_device
 
@@ -120,9 +123,24 @@ module mod_moloch
     type(c_ptr) function mlc_last_error() bind(c,name='mlc_last_error')
       import :: c_ptr
     end function mlc_last_error
+    integer(c_int) function mlc_output_open(h,dir,o) bind(c,name='mlc_output_open')
+      import :: c_int , c_ptr , c_char
+      type(c_ptr) , value :: h
+      character(kind=c_char) , dimension(*) , intent(in) :: dir
+      type(c_ptr) , intent(out) :: o
+    end function mlc_output_open
+    integer(c_int) function mlc_output_write(o) bind(c,name='mlc_output_write')
+      import :: c_int , c_ptr
+      type(c_ptr) , value :: o
+    end function mlc_output_write
+    integer(c_int) function mlc_output_close(o) bind(c,name='mlc_output_close')
+      import :: c_int , c_ptr
+      type(c_ptr) , value :: o
+    end function mlc_output_close
   end interface
 
   type(c_ptr) :: handle = c_null_ptr
+  type(c_ptr) :: output = c_null_ptr
 
   contains
 
@@ -307,6 +325,28 @@ module mod_moloch
     if ( c_associated(handle) ) call check(mlc_destroy(handle),'mlc_destroy')
     handle = c_null_ptr
   end subroutine moloch_finalize
+
+  ! Create ps.nc and t.nc in directory dir, as the driver's first
+  ! grid_nc_create calls do (nompi/regcm_dynamical_core.F90:205-208).
+  subroutine moloch_output_open(dir)
+    implicit none
+    character(len=*) , intent(in) :: dir
+    call check(mlc_output_open(handle,trim(dir)//c_null_char,output),'mlc_output_open')
+  end subroutine moloch_output_open
+
+  ! One record of ps and t as the device holds them now; the call returns
+  ! before the record reaches the files (grid_nc_write, :209-214).
+  subroutine moloch_output_write
+    implicit none
+    call check(mlc_output_write(output),'mlc_output_write')
+  end subroutine moloch_output_write
+
+  ! Wait for the queued records and close the files (grid_nc_destroy, :220-221).
+  subroutine moloch_output_close
+    implicit none
+    if ( c_associated(output) ) call check(mlc_output_close(output),'mlc_output_close')
+    output = c_null_ptr
+  end subroutine moloch_output_close
 
   ! moloch(istep, lprint) (nompi/mod_moloch.F90:272-421)
   subroutine moloch(istep,lprint)

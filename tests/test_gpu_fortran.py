@@ -54,6 +54,17 @@ def test_fortran_dropin_matches_oracle(tmp_path, prec, tol):
     got = _read(out)
     c = dm.case_config(0, jx=jx, iy=iy, kz=kz, kind=_lib.MLC_CASE_REFERENCE, hill_h=hill, hill_hw=hw)
     hs = dm.build_case(c)
+    # ps.nc / t.nc: records before steps 1 and 2 (iout = 2); the first is the
+    # initial state
+    from scipy.io import netcdf_file
+    with netcdf_file(str(tmp_path / "t.nc"), "r", mmap=False) as nc:
+        t = nc.variables["t"].data.copy()
+    with netcdf_file(str(tmp_path / "ps.nc"), "r", mmap=False) as nc:
+        ps = nc.variables["ps"].data.copy()
+    assert t.shape == (2, kz, iy, jx) and ps.shape == (2, iy, jx)
+    t0, ps0 = dm.interior(hs.fields["t"]), dm.interior(hs.fields["ps"])
+    assert compare(t[0], t0)[0] <= (0.0 if prec == 0 else 1e-6)
+    assert compare(ps[0], ps0)[0] <= (0.0 if prec == 0 else 1e-6)
     o = run_oracle(hs, dm.case_params(c), nsteps)
     for name, a in got.items():
         ref = dm.interior(o.field(name))
