@@ -114,6 +114,7 @@ EXPORTS = {
     "mlc_case_params": (C.c_int, [C.POINTER(Case), C.POINTER(Params)]),
     "mlc_case_build": (C.c_int, [C.POINTER(Case), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     "mlc_case_load": (C.c_int, [C.c_void_p, C.POINTER(Case)]),
+    "mlc_case_load_device": (C.c_int, [C.c_void_p, C.POINTER(Case)]),
     "mlc_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.c_int, C.POINTER(C.c_void_p)]),
     "mlc_mlp_destroy": (C.c_int, [C.c_void_p]),
     "mlc_mlp_weights": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),

@@ -152,6 +152,10 @@ class Moloch:
     def load_case(self, case: _lib.Case) -> None:
         check("mlc_case_load", load().mlc_case_load(self._h, C.byref(case)))
 
+    def load_case_device(self, case: _lib.Case) -> None:
+        """The case built on the device (3-D fields) instead of the host."""
+        check("mlc_case_load_device", load().mlc_case_load_device(self._h, C.byref(case)))
+
     def sync(self) -> None:
         check("mlc_sync", load().mlc_sync(self._h))
 

@@ -184,29 +184,34 @@ extern "C" int mlc_case_params(const mlc_case *c, mlc_params *p) {
   return MLC_OK;
 }
 
-extern "C" int mlc_case_build(const mlc_case *c, double *const *fields, double *const *levels) {
+// Levels, 2-D geometry and surface pressure of a case (canonical layout
+// (-1:jx+2, -1:iy+2)); *rng receives the generator state after the terrain
+// draws, where the theta noise continues.
+int case_host_part(const mlc_case *c, CaseHost &o) {
   if (!c) return mlc_fail(MLC_EINVAL, "mlc_case_build: NULL case");
   const int jx = c->jx, iy = c->iy, kz = c->kz;
   if (jx < 3 || iy < 3 || kz < 3 || kz > 1024)
     return mlc_fail(MLC_EINVAL, "mlc_case_build: need jx, iy >= 3 and 3 <= kz <= 1024");
   HostLayout L(jx, iy);
-  const long n2 = L.plane, n3 = L.plane * kz, n3p = L.plane * (kz + 1);
-  std::vector<double> zita(kz + 1), zitah(kz);
-
+  const long n2 = L.plane;
+  o.zita.assign(kz + 1, 0.0);
+  o.zitah.assign(kz, 0.0);
   // model_zitaf / model_zitah (mod_zita.F90:57-85)
   {
     double dz = c->ztop / (double)kz;
-    zita[kz] = 0.0;
-    for (int k = kz; k >= 1; k--) zita[k - 1] = zita[k] + dz;
-    zitah[kz - 1] = dz * 0.5;
-    for (int k = kz - 1; k >= 1; k--) zitah[k - 1] = zitah[k] + dz;
+    o.zita[kz] = 0.0;
+    for (int k = kz; k >= 1; k--) o.zita[k - 1] = o.zita[k] + dz;
+    o.zitah[kz - 1] = dz * 0.5;
+    for (int k = kz - 1; k >= 1; k--) o.zitah[k - 1] = o.zitah[k] + dz;
   }
-  const double dzita = zita[kz - 1];
-
-  std::vector<double> ht(n2, 0.0), hx(n2, 0.0), hy(n2, 0.0), msf(n2, 1.0), ulat(n2, 0.0), vlat(n2, 0.0);
-  std::vector<double> zeta(n3, 0.0), fmz(n3, 0.0), fmzf(n3p, 0.0), zabs(n3, 0.0);
-  std::vector<double> u(n3, 0.0), v(n3, 0.0), w(n3p, 0.0), pai(n3, 0.0), t(n3, 0.0), qv(n3, 0.0);
-  std::vector<double> ps(n2, 0.0);
+  o.ht.assign(n2, 0.0);
+  o.hx.assign(n2, 0.0);
+  o.hy.assign(n2, 0.0);
+  o.msf.assign(n2, 1.0);
+  o.ulat.assign(n2, 0.0);
+  o.vlat.assign(n2, 0.0);
+  o.ps.assign(n2, 0.0);
+  std::vector<double> &ht = o.ht, &hx = o.hx, &hy = o.hy, &msf = o.msf, &ulat = o.ulat, &vlat = o.vlat;
   Rng rng(c->seed);
 
   // terrain (geopotential, as `param` stores mddom%ht)
@@ -259,6 +264,36 @@ extern "C" int mlc_case_build(const mlc_case *c, double *const *fields, double *
     periodic_fill(L, hx.data(), 1);
     periodic_fill(L, hy.data(), 1);
   }
+  // surface pressure: fill_atmosphere's constant (mod_atmosphere.F90:184) or
+  // the standard atmosphere at the ground
+  if (c->kind == MLC_CASE_REFERENCE) {
+    for (long x = 0; x < n2; x++) o.ps[x] = 101480.0;
+  } else {
+    StdAtm A{c->tsfc, c->lapse, c->ttrop, c->psfc};
+    for (int i = 1; i <= iy; i++)
+      for (int j = 1; j <= jx; j++) o.ps[L.at(j, i)] = A.p(ht[L.at(j, i)] * kRegrav);
+    periodic_fill(L, o.ps.data(), 1);
+  }
+  o.rng = rng.s;
+  return MLC_OK;
+}
+
+extern "C" int mlc_case_build(const mlc_case *c, double *const *fields, double *const *levels) {
+  CaseHost H;
+  int rc = case_host_part(c, H);
+  if (rc) return rc;
+  const int jx = c->jx, iy = c->iy, kz = c->kz;
+  HostLayout L(jx, iy);
+  const long n2 = L.plane, n3 = L.plane * kz, n3p = L.plane * (kz + 1);
+  std::vector<double> &zita = H.zita, &zitah = H.zitah;
+  const double dzita = zita[kz - 1];
+  std::vector<double> &ht = H.ht, &hx = H.hx, &hy = H.hy, &msf = H.msf, &ulat = H.ulat, &vlat = H.vlat;
+  std::vector<double> zeta(n3, 0.0), fmz(n3, 0.0), fmzf(n3p, 0.0), zabs(n3, 0.0);
+  std::vector<double> u(n3, 0.0), v(n3, 0.0), w(n3p, 0.0), pai(n3, 0.0), t(n3, 0.0), qv(n3, 0.0);
+  std::vector<double> &ps = H.ps;
+  Rng rng(H.rng);
+  (void)n2;
+
   // zeta, fmz (:167-172), fmzf (:185-190)
 #pragma omp parallel for schedule(static)
   for (int k = 1; k <= kz; k++)
@@ -283,7 +318,7 @@ extern "C" int mlc_case_build(const mlc_case *c, double *const *fields, double *
     const double ts = 300.0, qs0 = 0.01865, ps0 = 101480.0, tropoh = 15000.0;
     double tvs = ts * (1.0 + kEp1 * qs0);
     double tvt = tvs - kLrate * tropoh;
-    for (long x = 0; x < n2; x++) ps[x] = ps0;
+    (void)ps0;
     for (int i = 1; i <= iy; i++)
       for (int j = 1; j <= jx; j++)
         for (int k = 1; k <= kz; k++) {
@@ -329,8 +364,6 @@ extern "C" int mlc_case_build(const mlc_case *c, double *const *fields, double *
           t[x] = A.T(z);
           qv[x] = c->rh * pfwsat_host(t[x], A.p(z));
         }
-    for (int i = 1; i <= iy; i++)
-      for (int j = 1; j <= jx; j++) ps[L.at(j, i)] = A.p(ht[L.at(j, i)] * kRegrav);
     // wind: u on the west face of cell j, heights averaged over j-1 and j
 #pragma omp parallel for schedule(static)
     for (int k = 1; k <= kz; k++)
@@ -410,7 +443,6 @@ extern "C" int mlc_case_build(const mlc_case *c, double *const *fields, double *
   periodic_fill(L, pai.data(), kz);
   periodic_fill(L, t.data(), kz);
   periodic_fill(L, qv.data(), kz);
-  periodic_fill(L, ps.data(), 1);
 
   auto put = [&](int f, const std::vector<double> &src) {
     if (fields && fields[f]) std::memcpy(fields[f], src.data(), src.size() * sizeof(double));

@@ -10,7 +10,18 @@ double pfwsat_host(double t, double p);                   // nompi/mod_moloch.F9
 
 // ------------------------------------------------------------ the MLP engine
 #include <hip/hip_runtime.h>
+#include <cstdint>
 #include <vector>
+
+// Host part of a case (levels, 2-D geometry, surface pressure) in the
+// canonical layout (-1:jx+2, -1:iy+2), and the state of the case's
+// splitmix64 generator after the terrain draws.
+struct mlc_case;
+struct CaseHost {
+  std::vector<double> zita, zitah, ht, hx, hy, msf, ulat, vlat, ps;
+  uint64_t rng = 0;
+};
+int case_host_part(const mlc_case *c, CaseHost &o);
 
 struct mlc_mlp {
   int nin, nhid, nout;

@@ -1657,6 +1657,177 @@ __global__ void __launch_bounds__(256) k_diag_final(double *partial, long nblk, 
     }
 }
 
+// ============================================================ case builder
+// The 3-D part of the case builder on the device (the host keeps the levels,
+// the 2-D geometry and ps): vertical coordinate (mod_zita.F90:87-179),
+// temperature, humidity and wind, the discrete hydrostatic Exner recursion
+// (mod_atmosphere.F90:210-234) and the perturbations, one column per lane,
+// all in FP64 as on the host; the splitmix64 draws of the theta noise are
+// taken by their index in the host's loop order.
+struct CaseDev {
+  int kind, noise_levels;
+  double ztop, zh, a0, tsfc, lapse, ttrop, psfc, rh, u0, v0, jet_amp, jet_z, jet_width;
+  double noise_amp, bubble_amp, bubble_rcells, bubble_z, bubble_rz, dzita;
+  unsigned long long rng;
+  const double *zita, *zitah, *ht, *ps;  // ht, ps in the device 2-D layout
+};
+
+__device__ __forceinline__ double cd_gzita(double z, const CaseDev &c) {
+  const double r = z / c.ztop;
+  return ((0.0 - 1.0 * c.a0) * r - (3.0 - 2.0 * c.a0) * (r * r) + (2.0 - 1.0 * c.a0) * (r * r * r)) + 1.0;
+}
+__device__ __forceinline__ double cd_zeta(double z, double geopot, const CaseDev &c) {
+  const double zfz = c.ztop / (exp(c.ztop / c.zh) - 1.0);
+  return geopot * kRegrav * (cd_gzita(z, c) - 1.0) + zfz * (exp(z / c.zh) - 1.0);
+}
+__device__ __forceinline__ double cd_fmz(double z, double geopot, const CaseDev &c) {
+  const double r = z / c.ztop, zfz = c.ztop / (exp(c.ztop / c.zh) - 1.0);
+  const double gp = ((0.0 - 1.0 * c.a0) * 1.0 - (6.0 - 4.0 * c.a0) * r + (6.0 - 3.0 * c.a0) * (r * r)) / c.ztop;
+  return 1.0 / (gp * (geopot * kRegrav) + zfz * exp(z / c.zh) / c.zh);
+}
+__device__ __forceinline__ double cd_T(double z, const CaseDev &c) { return fmax(c.tsfc - c.lapse * z, c.ttrop); }
+__device__ __forceinline__ double cd_p(double z, const CaseDev &c) {
+  const double expo = kEgrav / (kRgas * c.lapse), zt = (c.tsfc - c.ttrop) / c.lapse;
+  if (z <= zt) return c.psfc * pow(cd_T(z, c) / c.tsfc, expo);
+  return c.psfc * pow(c.ttrop / c.tsfc, expo) * exp(-kEgrav * (z - zt) / (kRgas * c.ttrop));
+}
+// temperature and humidity before the perturbations that need pai
+__device__ __forceinline__ void cd_tq(const CaseDev &c, const Geo &g, int j, int i, int k, double geopot, double &t,
+                                      double &q) {
+  const double zeta = cd_zeta(c.zitah[k - 1], geopot, c);
+  if (c.kind == MLC_CASE_REFERENCE) {
+    // fill_atmosphere (mod_atmosphere.F90:171-209)
+    const double ts = 300.0, qs0 = 0.01865, tropoh = 15000.0;
+    const double tvs = ts * (1.0 + kEp1 * qs0), tvt = tvs - kLrate * tropoh;
+    if (zeta < tropoh) {
+      q = qs0 * exp(-zeta / 4000.0) * exp((-zeta / 7500.0) * (-zeta / 7500.0));
+      t = (tvs - kLrate * zeta) / (1.0 + q);
+    } else {
+      q = 1.0e-8;
+      t = tvt;
+    }
+    if (k == g.nk) {  // the warm disc at the lowest level (:196-203), REAL() arithmetic
+      const float fjx = (float)g.nj, fiy = (float)g.ni;
+      const double aa = (double)fjx / 5.0;
+      const float bb = fiy / 5.0f, b2 = bb * bb;
+      const double mr = sqrt(aa * aa + (double)b2) / 2.0;
+      const float dj = (float)(j - g.nj / 2), di = (float)(i - g.ni / 2);
+      const double md = (double)sqrtf(dj * dj + di * di);
+      if (md < mr) t += (mr - md) / mr;
+    }
+  } else {
+    const double z = zeta + geopot * kRegrav;
+    t = cd_T(z, c);
+    q = c.rh * d_pfwsat(t, cd_p(z, c));
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_case_build(Dev<T> D, CaseDev c) {
+  COL_IDX
+  const int nk = g.nk;
+  const long c2 = g.at2(j, i);
+  const double geo = c.ht[c2];
+  // vertical coordinate, humidity, wind
+  for (int k = 1; k <= nk; k++) {
+    const long x = g.at(j, i, k);
+    D.zeta[x] = (T)cd_zeta(c.zitah[k - 1], geo, c);
+    D.fmz[x] = (T)cd_fmz(c.zitah[k - 1], geo, c);
+    double t, q;
+    cd_tq(c, g, j, i, k, geo, t, q);
+    D.qv[x] = (T)q;
+    double uu = 1.0, vv = 1.0;
+    if (c.kind != MLC_CASE_REFERENCE) {
+      // u on the west face of cell j: heights averaged over j-1 and j
+      const double gw = c.ht[g.at2(j - 1, i)];
+      const double zf = 0.5 * ((cd_zeta(c.zitah[k - 1], gw, c) + gw * kRegrav) +
+                               (cd_zeta(c.zitah[k - 1], geo, c) + geo * kRegrav));
+      uu = c.u0;
+      if (c.jet_amp != 0.0) {
+        const double e = (zf - c.jet_z) / c.jet_width;
+        uu += c.jet_amp * exp(-e * e);
+      }
+      vv = c.v0;
+    }
+    D.u[x] = (T)uu;
+    D.v[x] = (T)vv;
+  }
+  for (int k = 1; k <= nk + 1; k++) D.fmzf[g.at(j, i, k)] = (T)cd_fmz(c.zita[k - 1], geo, c);
+  // discrete hydrostatic Exner function, bottom up
+  double tk, qk;
+  cd_tq(c, g, j, i, nk, geo, tk, qk);
+  double pk;
+  {
+    double tm, qm;
+    cd_tq(c, g, j, i, nk - 1, geo, tm, qm);
+    const double zk = cd_zeta(c.zitah[nk - 1], geo, c), zm = cd_zeta(c.zitah[nk - 2], geo, c);
+    const double zdelta = zk * kEgrav;
+    const double tv1 = tk * (1.0 + kEp1 * qk), tv2 = tm * (1.0 + kEp1 * qm);
+    double lrt = (tv2 - tv1) / (zm - zk);
+    lrt = 0.65 * lrt - 0.35 * kLrate;
+    const double tv = tv1 - 0.5 * zk * lrt;
+    const double zz = 1.0 / (kRgas * tv);
+    const double p = c.ps[c2] * exp(-zdelta * zz);
+    pk = pow(p / kP00, kRovcp);
+    D.pai[g.at(j, i, nk)] = pk;
+  }
+  for (int k = nk - 1; k >= 1; k--) {
+    double t, q;
+    cd_tq(c, g, j, i, k, geo, t, q);
+    const double tv1 = t * (1.0 + kEp1 * q), tv2 = tk * (1.0 + kEp1 * qk);
+    const double ff = cd_fmz(c.zita[k], geo, c);  // fmzf at level k+1
+    const double zb = 2.0 * kEgrav * c.dzita / (ff * kCpd) + tv1 - tv2;
+    const double zdelta = sqrt(zb * zb + 4.0 * tv2 * tv1);
+    pk = -pk / (2.0 * tv2) * (zb - zdelta);
+    D.pai[g.at(j, i, k)] = pk;
+    tk = t;
+    qk = q;
+  }
+  // temperature with the perturbations applied as T' = theta' * pai
+  const int k0 = nk - c.noise_levels + 1 < 1 ? 1 : nk - c.noise_levels + 1;
+  const double jc = 0.5 * (double)(g.nj + 1), ic = 0.5 * (double)(g.ni + 1);
+  for (int k = 1; k <= nk; k++) {
+    const long x = g.at(j, i, k);
+    double t, q;
+    cd_tq(c, g, j, i, k, geo, t, q);
+    if (c.kind != MLC_CASE_REFERENCE) {
+      const double pk2 = D.pai[x];
+      if (c.noise_amp > 0.0 && k >= k0) {
+        const unsigned long long n = ((unsigned long long)(k - k0) * g.ni + (i - 1)) * g.nj + (j - 1);
+        unsigned long long z = c.rng + (n + 1) * 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const double uni = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+        t += (-c.noise_amp + (c.noise_amp - -c.noise_amp) * uni) * pk2;
+      }
+      if (c.bubble_amp != 0.0) {
+        const double dh = sqrt(((double)j - jc) * ((double)j - jc) + ((double)i - ic) * ((double)i - ic)) /
+                          c.bubble_rcells;
+        const double dz = (cd_zeta(c.zitah[k - 1], geo, c) - c.bubble_z) / c.bubble_rz;
+        const double r = sqrt(dh * dh + dz * dz);
+        if (r < 1.0) {
+          const double cc = cos(0.5 * kMathpi * r);
+          t += c.bubble_amp * cc * cc * pk2;
+        }
+      }
+    }
+    D.t[x] = (T)t;
+  }
+}
+
+// Every halo cell of the two-point ring from its periodic partner.
+template <typename F>
+__global__ void __launch_bounds__(256) k_periodic(F *f, Geo g) {
+  const int e = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y + 1;
+  const int wj = g.nj + 4, wi = g.ni + 4;
+  if (e >= wj * wi) return;
+  const int j = e % wj - 1, i = e / wj - 1;
+  if (j >= 1 && j <= g.nj && i >= 1 && i <= g.ni) return;
+  const int jj = ((j - 1) % g.nj + g.nj) % g.nj + 1, ii = ((i - 1) % g.ni + g.ni) % g.ni + 1;
+  f[g.at(j, i, k)] = f[g.at(jj, ii, k)];
+}
+
 // ============================================================ output packing
 // Interior points of a field, j fastest, as doubles: the layout of the
 // reference's output records (nompi/mod_mppparam.F90:197,310).
@@ -1701,6 +1872,7 @@ struct CoreBase {
   // recorded after them; wait_event makes the stream wait for an event
   virtual int pack_output(double *dst, hipEvent_t done) = 0;
   virtual int wait_event(hipEvent_t e) = 0;
+  virtual int build_case(const mlc_case &c, const CaseHost &H) = 0;
 };
 
 static long round_up(long a, long b) { return (a + b - 1) / b * b; }
@@ -2635,6 +2807,57 @@ struct Core : CoreBase {
     return MLC_OK;
   }
 
+  // 3-D fields of a case on the device; the 2-D fields and the levels have
+  // been set from H.
+  int build_case(const mlc_case &cs, const CaseHost &H) override {
+    const int K = g.nk;
+    const long n2h = (long)(g.nj + 4) * (g.ni + 4);
+    double *d2 = nullptr;  // ht, ps in the device 2-D layout, zita, zitah
+    const size_t n2 = (size_t)g.plane;
+    HIP_TRY(hipMalloc((void **)&d2, (2 * n2 + 2 * (K + 1)) * sizeof(double)));
+    std::vector<double> h2(2 * n2 + 2 * (K + 1), 0.0);
+    for (int i = -1; i <= g.ni + 2; i++)
+      for (int j = -1; j <= g.nj + 2; j++) {
+        const long hh = (long)(i + 1) * (g.nj + 4) + (j + 1);
+        h2[g.at2(j, i)] = H.ht[hh];
+        h2[n2 + g.at2(j, i)] = H.ps[hh];
+      }
+    (void)n2h;
+    for (int k = 0; k <= K; k++) h2[2 * n2 + k] = H.zita[k];
+    for (int k = 0; k < K; k++) h2[2 * n2 + K + 1 + k] = H.zitah[k];
+    hipError_t e = hipMemcpyAsync(d2, h2.data(), h2.size() * sizeof(double), hipMemcpyHostToDevice, stream);
+    CaseDev c{};
+    c.kind = cs.kind;
+    c.noise_levels = cs.noise_levels;
+    c.ztop = cs.ztop, c.zh = cs.zh, c.a0 = cs.a0, c.tsfc = cs.tsfc, c.lapse = cs.lapse, c.ttrop = cs.ttrop;
+    c.psfc = cs.psfc, c.rh = cs.rh, c.u0 = cs.u0, c.v0 = cs.v0, c.jet_amp = cs.jet_amp, c.jet_z = cs.jet_z;
+    c.jet_width = cs.jet_width, c.noise_amp = cs.noise_amp, c.bubble_amp = cs.bubble_amp;
+    c.bubble_rcells = cs.bubble_rcells, c.bubble_z = cs.bubble_z, c.bubble_rz = cs.bubble_rz;
+    c.dzita = H.zita[K - 1];
+    c.rng = H.rng;
+    c.ht = d2;
+    c.ps = d2 + n2;
+    c.zita = d2 + 2 * n2;
+    c.zitah = d2 + 2 * n2 + K + 1;
+    // fields the case leaves at zero
+    T *zf[] = {D.w, D.tetav, D.qc, D.qi, D.qr, D.qs, D.tvirt, D.p, D.rho, D.qsat, D.ux, D.vx};
+    const int zl[] = {K + 1, K, K, K, K, K, K, K, K, K, K, K};
+    for (int f = 0; f < 12 && e == hipSuccess; f++) e = hipMemsetAsync(zf[f], 0, g.plane * zl[f] * sizeof(T), stream);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_case_build<T>, col_grid(), blk(), 0, stream, D, c);
+      const dim3 pg((unsigned)(((g.nj + 4) * (g.ni + 4) + 255) / 256), 1);
+      T *pf[] = {D.u, D.v, D.t, D.qv, D.zeta, D.fmz};
+      for (T *f : pf) hipLaunchKernelGGL(k_periodic<T>, dim3(pg.x, K), dim3(256), 0, stream, f, g);
+      hipLaunchKernelGGL(k_periodic<T>, dim3(pg.x, K + 1), dim3(256), 0, stream, D.fmzf, g);
+      hipLaunchKernelGGL(k_periodic<double>, dim3(pg.x, K), dim3(256), 0, stream, D.pai, g);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    (void)hipFree(d2);
+    if (e != hipSuccess) return mlc_fail(MLC_EHIP, "mlc_case_load_device: %s", hipGetErrorString(e));
+    return MLC_OK;
+  }
+
   int lac_dataset(long first, int n, float *X, float *Tt, int *cols) override {
     if (!L.exists) return mlc_fail(MLC_ESTATE, "mlc_lac_dataset: no corrector (mlc_lac_create)");
     if (first < 0 || n < 0 || first + n > L.count) return mlc_fail(MLC_EINVAL, "mlc_lac_dataset: rows out of range");
@@ -3091,4 +3314,26 @@ extern "C" int mlc_output_close(mlc_output *o) {
   int rc = mlc_output_flush(o);
   output_release(o);
   return rc;
+}
+
+extern "C" int mlc_case_load_device(mlc_handle *h, const mlc_case *c) {
+  if (!h || !c) return mlc_fail(MLC_EINVAL, "mlc_case_load_device: NULL argument");
+  if (c->jx != h->dims.jx || c->iy != h->dims.iy || c->kz != h->dims.kz)
+    return mlc_fail(MLC_EINVAL, "mlc_case_load_device: case grid differs from the handle's");
+  CaseHost H;
+  int rc = case_host_part(c, H);
+  if (rc) return rc;
+  const int jx = c->jx, iy = c->iy, kz = c->kz;
+  int lb[3] = {-1, -1, 1}, ub[3] = {jx + 2, iy + 2, 1};
+  const std::pair<int, const std::vector<double> *> two[] = {
+      {MLC_PS, &H.ps}, {MLC_HX, &H.hx}, {MLC_HY, &H.hy}, {MLC_MSFU, &H.msf}, {MLC_MSFV, &H.msf},
+      {MLC_MSFX, &H.msf}, {MLC_ULAT, &H.ulat}, {MLC_VLAT, &H.vlat}};
+  for (auto &f : two)
+    if ((rc = mlc_set_field(h, f.first, f.second->data(), lb, ub))) return rc;
+  std::vector<double> ffilt(kz, 0.0);
+  if ((rc = mlc_set_levels(h, MLC_ZITA, H.zita.data(), kz + 1))) return rc;
+  if ((rc = mlc_set_levels(h, MLC_ZITAH, H.zitah.data(), kz))) return rc;
+  if ((rc = mlc_set_levels(h, MLC_FFILT, ffilt.data(), kz))) return rc;
+  if ((rc = h->core->build_case(*c, H))) return rc;
+  return mlc_init(h);
 }

@@ -246,6 +246,12 @@ int mlc_case_build(const mlc_case *c, double *const *fields, double *const *leve
  * mlc_init. */
 int mlc_case_load(mlc_handle *h, const mlc_case *c);
 
+/* The same case built on the device: the levels, the 2-D geometry and ps on
+ * the host as in mlc_case_build, the 3-D fields by a kernel straight into
+ * the handle's arrays, then mlc_init.  The 3-D fields agree with
+ * mlc_case_build to the last few bits of the device exp/pow (FP64). */
+int mlc_case_load_device(mlc_handle *h, const mlc_case *c);
+
 #ifdef __cplusplus
 }
 #endif
