@@ -233,8 +233,11 @@ def main() -> int:
     chosen = results[chosen_tag]
     traffic = traffic_table(jx, iy, kz, "fp64" if chosen_tag == "fp64" else "fp32")
     roofline = roofline_of(chosen["_prof"], dominant(chosen["_prof"])[0], traffic)
-    stencil = roofline_of(chosen["_prof"], max((k for k in chosen["_prof"] if k not in MFMA_KINDS),
-                                               key=lambda k: chosen["_prof"][k][0]), traffic)
+    # the core's dominant stencil kernel (the column solve), and every stencil
+    # kind with its algorithmic and measured rates
+    stencil = roofline_of(chosen["_prof"], "sound_b", traffic)
+    stencils = {k: roofline_of(chosen["_prof"], k, traffic) for k in chosen["_prof"]
+                if k not in MFMA_KINDS and chosen["_prof"][k][2] > 0}
     mlp = roofline_of(chosen["_prof"], "lac_mlp", traffic) if "lac_mlp" in chosen["_prof"] else None
 
     cpu = None
@@ -277,6 +280,8 @@ def main() -> int:
         "rel_l2_after_steps": args.lead + args.warmup + args.steps + 1,
         "roofline": roofline,
         "stencil_roofline": stencil,
+        "stencil_kernels": {k: {x: r[x] for x in ("achieved", "frac", "traffic", "avg_launch_ms", "measured_gbs")
+                                if x in r} for k, r in stencils.items()},
         "mlp_roofline": mlp,
         "cpu_baseline": cpu,
         "variants": results,
