@@ -839,236 +839,219 @@ __global__ void __launch_bounds__(256) k_wafv(Dev<T> D, Par P, FieldSet<Q> F, do
   }
 }
 
-// Horizontal sweeps of one level on a tile of HJ x HI cells.
-constexpr int HJ = 60, HI = 12;
-constexpr int HEJ = HJ + 4, HEI = HI + 4;  // 64 x 16 with the reach of both sweeps
+// Horizontal sweeps of one level, one wave per strip of HJ columns x HR rows:
+// the wave's 64 lanes are the strip's columns plus the two-column reach of
+// the zonal sweep on either side.
+constexpr int HJ = 60, HR = 32;
 
-// Block index -> tile, grouping consecutive tiles on one XCD: the hardware
-// hands consecutive workgroups to the eight XCDs in turn, and neighbouring
-// tiles share halo lines through that XCD's L2.
+// Block index -> work item, grouping consecutive items on one XCD: the
+// hardware hands consecutive workgroups to the eight XCDs in turn, and
+// neighbouring strips share halo lines through that XCD's L2.
 __device__ __forceinline__ long xcd_remap(long b, long total) {
   const long nx = 8;
   long chunk = (total + nx - 1) / nx;
   return (b % nx) * chunk + b / nx;
 }
 
-// One (tile, level) per block; for each field of the group: meridional face
-// fluxes -> p0 on the rows of the extended tile -> zonal face fluxes ->
-// result.  Everything that does not depend on the advected field (Courant
-// numbers, fmz ratios, the divergence-correction coefficients) is evaluated
-// once per block; each face flux once per field.
+// Periodic index in 1..n of any integer.
+__device__ __forceinline__ int wrap_idx(int x, int n) { return ((x - 1) % n + n) % n + 1; }
+
+// For each field of the group the rows of the strip are swept south to
+// north with a window of five rows of wz (rows r-2 .. r+2) in registers:
+// the meridional flux at the north face of row r (:1082-1100) comes from the
+// window, p0 of row r (:1104-1131) from that flux and the one carried from
+// row r-1, the zonal face fluxes (:1138-1160) and the result (:1162-1187)
+// from p0 of the neighbouring lanes.  No block barrier; the loads of row
+// r+1 are issued before row r is evaluated.  Everything that does not depend
+// on the advected field (Courant numbers, fmz ratios, the divergence-
+// correction coefficients) is evaluated once per row for all fields.
 template <typename T, typename Q, int NF, bool ROT>
 __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, double dta, int ntj, int nti) {
-  __shared__ Q sWz[HEI][HEJ];       // wz, rows i0-2 .. i0+HI+1
-  __shared__ Q sPp[2][HI][HEJ];     // field, rows i0 .. i0+HI-1 (double buffered)
-  __shared__ Q sZp[HI + 1][HEJ];    // meridional flux at the south face of rows i0 .. i0+HI
-  __shared__ Q sP0[HI][HEJ];        // p0
-  __shared__ Q sZw[HI][HJ + 1];     // zonal flux at the west face of columns j0 .. j0+HJ
   const Geo &g = D.g;
   const int nj = g.nj, ni = g.ni;
-  const long total = (long)ntj * nti * g.nk;
-  const long lb = xcd_remap((long)blockIdx.x, total);
-  if (lb >= total) return;
-  const int tj = (int)(lb % ntj), ti = (int)((lb / ntj) % nti), k = (int)(lb / ((long)ntj * nti)) + 1;
-  const int j0 = tj * HJ + 1, i0 = ti * HI + 1;
-  const int tid = threadIdx.x, ej = tid % HEJ, rr = tid / HEJ;
+  const long total = (long)ntj * nti * g.nk;  // strips
+  const long nblk = (total + 3) / 4;
+  const long lb = xcd_remap((long)blockIdx.x, nblk);
+  if (lb >= nblk) return;
+  const long item = lb * 4 + (threadIdx.x >> 6);
+  if (item >= total) return;  // the whole wave
+  const int lane = threadIdx.x & 63;
+  const int tj = (int)(item % ntj), ti = (int)((item / ntj) % nti), k = (int)(item / ((long)ntj * nti)) + 1;
+  const int j0 = tj * HJ + 1, r0 = ti * HR + 1;
+  const int nrows = ni - r0 + 1 < HR ? ni - r0 + 1 : HR;
+  const int ja = j0 - 2 + lane;  // the lane's column (unwrapped)
+  const bool store = lane >= 2 && lane < 2 + HJ && ja <= nj;
+  const int jw = wrap_idx(ja, nj);
+  const int jp = jw == nj ? 1 : jw + 1, jm = jw == 1 ? nj : jw - 1;
   const Q dtrdx = (Q)(dta / P.dx), dtrdy = (Q)(dta / P.dx);
-  auto wj = [&](int j) { return j < 1 ? j + nj : (j > nj ? j - nj : j); };
-  auto wi = [&](int i) { return i < 1 ? i + ni : (i > ni ? i - ni : i); };
-  const int jw = wj(j0 - 2 + ej);
+  const long pl = (long)(k - 1) * g.plane;
+  auto at3 = [&](int jj, int ii) { return pl + (long)(ii + 1) * g.pj + (long)(jj + JOFF); };
+  auto nxt = [&](int ii) { return ii == ni ? 1 : ii + 1; };
+  const int lm1 = lane > 0 ? lane - 1 : 0, lm2 = lane > 1 ? lane - 2 : 0, lp1 = lane < 63 ? lane + 1 : 63;
 
-  // ---- field-independent parts
-  // meridional faces f = rr + 4n (south face of row i0+f), n = 0..3
-  Q mz[4], ma[4];
-#pragma unroll
-  for (int n = 0; n < 4; n++) {
-    const int f = rr + 4 * n;
-    mz[n] = Q(0);
-    ma[n] = Q(0);
-    if (f <= HI) {
-      const int iw = wi(i0 + f);
-      Q vv = (Q)D.v[g.at(jw, iw, k)];
-      if constexpr (ROT) {
-        mz[n] = vv * dtrdy;
-        ma[n] = Q(0.5) * vv;
-      } else {
-        Q rm = (Q)D.rmv[g.at2(jw, iw)];
-        mz[n] = vv * rm * dtrdy;
-        ma[n] = Q(0.5) * vv * rm;
-      }
-    }
-  }
-  // p0 points r = rr + 4n, n = 0..2
-  Q pfs[3], pfn[3], pcd[3], pm2[3];
-#pragma unroll
-  for (int n = 0; n < 3; n++) {
-    const int r = rr + 4 * n;
-    const int iw = wi(i0 + r), ip = wi(i0 + r + 1), im = wi(i0 + r - 1);
-    Q f0 = (Q)D.fmz[g.at(jw, iw, k)], fn = (Q)D.fmz[g.at(jw, ip, k)], fs = (Q)D.fmz[g.at(jw, im, k)];
-    Q vc = (Q)D.v[g.at(jw, iw, k)], vp = (Q)D.v[g.at(jw, ip, k)];
-    if constexpr (ROT) {
-      Q mx = (Q)D.mx[g.at2(jw, iw)];
-      Q zhxvtn = dtrdy * (Q)D.rmv[g.at2(jw, ip)] * mx;
-      Q zhxvts = dtrdy * (Q)D.rmv[g.at2(jw, iw)] * mx;
-      pfn[n] = zhxvtn * Q(2) * f0 / (f0 + fn);
-      pfs[n] = zhxvts * Q(2) * f0 / (f0 + fs);
-      pcd[n] = vp * pfn[n] - vc * pfs[n];
-      pm2[n] = Q(1);
+  // meridional flux through the south face of row c (Courant number mz,
+  // amplitude ma) from wz of rows c-2, c-1, c, c+1
+  auto mflux = [&](Q mz, Q ma, Q wmm, Q wm, Q wc, Q wp) -> Q {
+    Q is, a, b;
+    if (mz > Q(0)) {
+      is = Q(1);
+      a = wm;
+      b = wmm;
     } else {
-      pfn[n] = dtrdy * Q(2) * f0 / (f0 + fn);
-      pfs[n] = dtrdy * Q(2) * f0 / (f0 + fs);
-      pcd[n] = vp * (Q)D.rmv[g.at2(jw, ip)] * pfn[n] - vc * (Q)D.rmv[g.at2(jw, iw)] * pfs[n];
-      pm2[n] = (Q)D.mx2[g.at2(jw, iw)];
+      is = Q(-1);
+      a = wp;
+      b = wc;
     }
-  }
-  // zonal faces q = tid + 256n over (jf, r) in (HJ+1) x HI, n = 0..2
-  Q zz[3], za[3];
-#pragma unroll
-  for (int n = 0; n < 3; n++) {
-    const int q = tid + 256 * n;
-    zz[n] = Q(0);
-    za[n] = Q(0);
-    if (q < (HJ + 1) * HI) {
-      const int jf = q % (HJ + 1), r = q / (HJ + 1);
-      const int jw2 = wj(j0 + jf), iw = wi(i0 + r);
-      Q uu = (Q)D.u[g.at(jw2, iw, k)];
-      if constexpr (ROT) {
-        zz[n] = uu * (Q)D.mu[g.at2(jw2, iw)] * dtrdx;
-        za[n] = Q(0.5) * uu;
-      } else {
-        Q rm = (Q)D.rmu[g.at2(jw2, iw)];
-        zz[n] = uu * rm * dtrdx;
-        za[n] = Q(0.5) * uu * rm;
-      }
-    }
-  }
-  // zonal points q = tid + 256n over (jt, r) in HJ x HI, n = 0..2
-  Q zfw[3], zfe[3], zcd[3], zm2[3];
-#pragma unroll
-  for (int n = 0; n < 3; n++) {
-    const int q = tid + 256 * n;
-    zfw[n] = zfe[n] = zcd[n] = zm2[n] = Q(0);
-    if (q < HJ * HI) {
-      const int jt = q % HJ, r = q / HJ;
-      const int j = wj(j0 + jt), jp = wj(j0 + jt + 1), jm = wj(j0 + jt - 1), iw = wi(i0 + r);
-      Q f0 = (Q)D.fmz[g.at(j, iw, k)], fe = (Q)D.fmz[g.at(jp, iw, k)], fw = (Q)D.fmz[g.at(jm, iw, k)];
-      Q uc = (Q)D.u[g.at(j, iw, k)], up = (Q)D.u[g.at(jp, iw, k)];
-      if constexpr (ROT) {
-        Q zcostx = dtrdx * (Q)D.mu[g.at2(j, iw)];
-        zfe[n] = zcostx * Q(2) * f0 / (f0 + fe);
-        zfw[n] = zcostx * Q(2) * f0 / (f0 + fw);
-        zcd[n] = up * zfe[n] - uc * zfw[n];
-        zm2[n] = Q(1);
-      } else {
-        zfe[n] = dtrdx * Q(2) * f0 / (f0 + fe);
-        zfw[n] = dtrdx * Q(2) * f0 / (f0 + fw);
-        zcd[n] = up * (Q)D.rmu[g.at2(jp, iw)] * zfe[n] - uc * (Q)D.rmu[g.at2(j, iw)] * zfw[n];
-        zm2[n] = (Q)D.mx2[g.at2(j, iw)];
-      }
-    }
-  }
-
-  // ---- field loop, loads of the next field issued ahead
-  Q lw[4], lp[3];
-  auto fetch = [&](int f) {
-#pragma unroll
-    for (int n = 0; n < 4; n++) lw[n] = F.wz[f][g.at(jw, wi(i0 - 2 + rr + 4 * n), k)];
-#pragma unroll
-    for (int n = 0; n < 3; n++) lp[n] = F.in[f][g.at(jw, wi(i0 + rr + 4 * n), k)];
+    const Q zzden = sgn_floor(wc - wm);
+    const Q rq = (a - b) / zzden;
+    const Q bb = waf_b(rq);
+    const Q zphi = is + mz * bb - is * bb;
+    return ma * ((Q(1) + zphi) * wm + (Q(1) - zphi) * wc);
   };
-  fetch(0);
+  // face coefficients of the south face of row ii
+  auto mface = [&](int ii, Q &mz, Q &ma) {
+    const Q vv = (Q)D.v[at3(jw, ii)];
+    if constexpr (ROT) {
+      mz = vv * dtrdy;
+      ma = Q(0.5) * vv;
+    } else {
+      const Q rm = (Q)D.rmv[g.at2(jw, ii)];
+      mz = vv * rm * dtrdy;
+      ma = Q(0.5) * vv * rm;
+    }
+  };
+
+  // rows r-2 .. r+2 of the first row r = r0
+  int i_m2 = wrap_idx(r0 - 2, ni);
+  int i_m1 = nxt(i_m2), i_c = nxt(i_m1), i_p1 = nxt(i_c), i_p2 = nxt(i_p1);
+  Q w[NF][5], zs[NF];
+  {
+    Q mz, ma;
+    mface(i_c, mz, ma);
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      w[f][0] = F.wz[f][at3(jw, i_m2)];
+      w[f][1] = F.wz[f][at3(jw, i_m1)];
+      w[f][2] = F.wz[f][at3(jw, i_c)];
+      w[f][3] = F.wz[f][at3(jw, i_p1)];
+      zs[f] = mflux(mz, ma, w[f][0], w[f][1], w[f][2], w[f][3]);
+    }
+  }
+  // loads of the first row
+  Q nw[NF], np[NF];
+#pragma unroll
   for (int f = 0; f < NF; f++) {
-    const int pb = f & 1;
+    nw[f] = F.wz[f][at3(jw, i_p2)];
+    np[f] = F.in[f][at3(jw, i_c)];
+  }
+  for (int r = 0; r < nrows; r++) {
+    Q pp[NF];
 #pragma unroll
-    for (int n = 0; n < 4; n++) sWz[rr + 4 * n][ej] = lw[n];
+    for (int f = 0; f < NF; f++) {
+      w[f][4] = nw[f];
+      pp[f] = np[f];
+    }
+    if (r + 1 < nrows) {  // uniform over the wave
+      const int i_n = nxt(i_p2);
 #pragma unroll
-    for (int n = 0; n < 3; n++) sPp[pb][rr + 4 * n][ej] = lp[n];
-    __syncthreads();
-    if (f + 1 < NF) fetch(f + 1);
-    // meridional face fluxes (:1082-1100)
-#pragma unroll
-    for (int n = 0; n < 4; n++) {
-      const int fr = rr + 4 * n;
-      if (fr <= HI) {
-        Q zamu = mz[n], is;
-        int oh, ohm1;  // rows of ih and ihm1 relative to the face's cell row
-        if (zamu > Q(0)) {
-          is = Q(1);
-          oh = -1;
-          ohm1 = -2;
-        } else {
-          is = Q(-1);
-          oh = 1;
-          ohm1 = 0;
-        }
-        const int er = fr + 2;  // row of the cell north of the face
-        Q wc = sWz[er][ej], wm = sWz[er - 1][ej];
-        Q zzden = sgn_floor(wc - wm);
-        Q rq = (sWz[er + oh][ej] - sWz[er + ohm1][ej]) / zzden;
-        Q bb = waf_b(rq);
-        Q zphi = is + zamu * bb - is * bb;
-        sZp[fr][ej] = ma[n] * ((Q(1) + zphi) * wm + (Q(1) - zphi) * wc);
+      for (int f = 0; f < NF; f++) {
+        nw[f] = F.wz[f][at3(jw, i_n)];
+        np[f] = F.in[f][at3(jw, i_p1)];
       }
     }
-    __syncthreads();
-    // p0 (:1104-1131)
+    // field-independent coefficients of row i_c
+    Q mzn, man;  // north face (south face of row i_p1)
+    mface(i_p1, mzn, man);
+    Q pfs, pfn, pcd, pm2;
+    {
+      const Q f0 = (Q)D.fmz[at3(jw, i_c)], fn = (Q)D.fmz[at3(jw, i_p1)], fs = (Q)D.fmz[at3(jw, i_m1)];
+      const Q vc = (Q)D.v[at3(jw, i_c)], vp = (Q)D.v[at3(jw, i_p1)];
+      if constexpr (ROT) {
+        const Q mx = (Q)D.mx[g.at2(jw, i_c)];
+        const Q zhxvtn = dtrdy * (Q)D.rmv[g.at2(jw, i_p1)] * mx;
+        const Q zhxvts = dtrdy * (Q)D.rmv[g.at2(jw, i_c)] * mx;
+        pfn = zhxvtn * Q(2) * f0 / (f0 + fn);
+        pfs = zhxvts * Q(2) * f0 / (f0 + fs);
+        pcd = vp * pfn - vc * pfs;
+        pm2 = Q(1);
+      } else {
+        pfn = dtrdy * Q(2) * f0 / (f0 + fn);
+        pfs = dtrdy * Q(2) * f0 / (f0 + fs);
+        pcd = vp * (Q)D.rmv[g.at2(jw, i_p1)] * pfn - vc * (Q)D.rmv[g.at2(jw, i_c)] * pfs;
+        pm2 = (Q)D.mx2[g.at2(jw, i_c)];
+      }
+    }
+    Q zz, za, zfw, zfe, zcd, zm2;
+    {
+      const Q uc = (Q)D.u[at3(jw, i_c)], up = (Q)D.u[at3(jp, i_c)];
+      const Q f0 = (Q)D.fmz[at3(jw, i_c)], fe = (Q)D.fmz[at3(jp, i_c)], fw = (Q)D.fmz[at3(jm, i_c)];
+      if constexpr (ROT) {
+        const Q mu = (Q)D.mu[g.at2(jw, i_c)];
+        zz = uc * mu * dtrdx;
+        za = Q(0.5) * uc;
+        const Q zcostx = dtrdx * mu;
+        zfe = zcostx * Q(2) * f0 / (f0 + fe);
+        zfw = zcostx * Q(2) * f0 / (f0 + fw);
+        zcd = up * zfe - uc * zfw;
+        zm2 = Q(1);
+      } else {
+        const Q rm = (Q)D.rmu[g.at2(jw, i_c)];
+        zz = uc * rm * dtrdx;
+        za = Q(0.5) * uc * rm;
+        zfe = dtrdx * Q(2) * f0 / (f0 + fe);
+        zfw = dtrdx * Q(2) * f0 / (f0 + fw);
+        zcd = up * (Q)D.rmu[g.at2(jp, i_c)] * zfe - uc * rm * zfw;
+        zm2 = (Q)D.mx2[g.at2(jw, i_c)];
+      }
+    }
+    const long xo = at3(ja <= nj ? ja : jw, i_c);
 #pragma unroll
-    for (int n = 0; n < 3; n++) {
-      const int r = rr + 4 * n;
-      Q zdv = pcd[n] * sPp[pb][r][ej];
+    for (int f = 0; f < NF; f++) {
+      // meridional: flux at the north face, then p0 of row i_c
+      const Q zn = mflux(mzn, man, w[f][1], w[f][2], w[f][3], w[f][4]);
+      Q p0;
+      const Q zdv = pcd * pp[f];
+      if constexpr (ROT)
+        p0 = w[f][2] + zs[f] * pfs - zn * pfn + zdv;
+      else
+        p0 = w[f][2] + pm2 * (zs[f] * pfs - zn * pfn + zdv);
+      // zonal: flux at the west face of the lane's column, then the result
+      const Q pm = __shfl(p0, lm1, 64), pmm = __shfl(p0, lm2, 64), ppl = __shfl(p0, lp1, 64);
+      Q is, a, b;
+      if (zz > Q(0)) {
+        is = Q(1);
+        a = pm;
+        b = pmm;
+      } else {
+        is = Q(-1);
+        a = ppl;
+        b = p0;
+      }
+      const Q zzden = sgn_floor(p0 - pm);
+      const Q rq = (a - b) / zzden;
+      const Q bb = waf_b(rq);
+      const Q zphi = is + zz * bb - is * bb;
+      const Q zw = za * ((Q(1) + zphi) * pm + (Q(1) - zphi) * p0);
+      const Q ze = __shfl(zw, lp1, 64);
+      const Q zdu = zcd * pp[f];
       Q o;
       if constexpr (ROT)
-        o = sWz[r + 2][ej] + sZp[r][ej] * pfs[n] - sZp[r + 1][ej] * pfn[n] + zdv;
+        o = p0 + zw * zfw - ze * zfe + zdu;
       else
-        o = sWz[r + 2][ej] + pm2[n] * (sZp[r][ej] * pfs[n] - sZp[r + 1][ej] * pfn[n] + zdv);
-      sP0[r][ej] = o;
+        o = p0 + zm2 * (zw * zfw - ze * zfe + zdu);
+      if (store) F.out[f][xo] = o;
+      // slide the window
+      w[f][0] = w[f][1];
+      w[f][1] = w[f][2];
+      w[f][2] = w[f][3];
+      w[f][3] = w[f][4];
+      zs[f] = zn;
     }
-    __syncthreads();
-    // zonal face fluxes (:1138-1160)
-#pragma unroll
-    for (int n = 0; n < 3; n++) {
-      const int q = tid + 256 * n;
-      if (q < (HJ + 1) * HI) {
-        const int jf = q % (HJ + 1), r = q / (HJ + 1);
-        Q zamu = zz[n], is;
-        int oh, ohm1;
-        if (zamu > Q(0)) {
-          is = Q(1);
-          oh = -1;
-          ohm1 = -2;
-        } else {
-          is = Q(-1);
-          oh = 1;
-          ohm1 = 0;
-        }
-        const int e = jf + 2;  // extended column of the cell east of the face
-        Q pc = sP0[r][e], pm = sP0[r][e - 1];
-        Q zzden = sgn_floor(pc - pm);
-        Q rq = (sP0[r][e + oh] - sP0[r][e + ohm1]) / zzden;
-        Q bb = waf_b(rq);
-        Q zphi = is + zamu * bb - is * bb;
-        sZw[r][jf] = za[n] * ((Q(1) + zphi) * pm + (Q(1) - zphi) * pc);
-      }
-    }
-    __syncthreads();
-    // result (:1162-1187)
-#pragma unroll
-    for (int n = 0; n < 3; n++) {
-      const int q = tid + 256 * n;
-      if (q < HJ * HI) {
-        const int jt = q % HJ, r = q / HJ;
-        const int j = j0 + jt, i = i0 + r;
-        if (j <= nj && i <= ni) {
-          Q zdv = zcd[n] * sPp[pb][r][jt + 2];
-          Q o;
-          if constexpr (ROT)
-            o = sP0[r][jt + 2] + sZw[r][jt] * zfw[n] - sZw[r][jt + 1] * zfe[n] + zdv;
-          else
-            o = sP0[r][jt + 2] + zm2[n] * (sZw[r][jt] * zfw[n] - sZw[r][jt + 1] * zfe[n] + zdv);
-          F.out[f][g.at(j, i, k)] = o;
-        }
-      }
-    }
+    i_m2 = i_m1;
+    i_m1 = i_c;
+    i_c = i_p1;
+    i_p1 = i_p2;
+    i_p2 = nxt(i_p2);
   }
 }
 
@@ -2323,8 +2306,8 @@ struct Core : CoreBase {
     launch(KK_WAF_V, fbytes(3, 0) + (q ? fbytes(0, NF) : fbytes(NF, 0)), k_wafv<T, Q, NF>, col_grid(),
            blk(), D, P, F, dta);
     // horizontal: u, v, fmz + NF x (wz, field) -> NF results
-    const int ntj = (g.nj + HJ - 1) / HJ, nti = (g.ni + HI - 1) / HI;
-    const long total = (long)ntj * nti * g.nk;
+    const int ntj = (g.nj + HJ - 1) / HJ, nti = (g.ni + HR - 1) / HR;
+    const long total = ((long)ntj * nti * g.nk + 3) / 4;  // blocks of four strips
     if (prm.rotllr)
       launch(KK_WAF_H, q ? fbytes(0, NF) : fbytes(NF, 0), k_wafh<T, Q, NF, true>,
              dim3((unsigned)round_up(total, 8)), dim3(256), D, P, F, dta, ntj, nti);

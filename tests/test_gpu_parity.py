@@ -149,17 +149,27 @@ def test_fp32_error_budget():
         assert rel < budget[n], n
 
 
+FUSION_OPTS = [{}, {"divfilter": 1}, {"rotllr": 0}, {"halo_mpi": 1}]
+
+
 @pytest.mark.parametrize("prec", [dm.MLC_FP64, dm.MLC_FP32])
-@pytest.mark.parametrize("grid", [(70, 45, 20), (29, 13, 7), (3, 4, 3)])
-def test_fused_advection_equals_unfused(prec, grid):
-    """The fused wafone kernel performs the reference's arithmetic in the
-    reference's order: bit-identical to the three-sweep kernels."""
+@pytest.mark.parametrize("grid", [(70, 45, 20), (29, 13, 7), (3, 4, 3), (130, 9, 5)])
+@pytest.mark.parametrize("opts", range(len(FUSION_OPTS)))
+def test_fused_equals_unfused(prec, grid, opts):
+    """The two-kernel wafone (vertical passes, then both horizontal sweeps in
+    one strip kernel) performs the reference's arithmetic in the reference's
+    order: bit-identical to the three-sweep kernels, on grids smaller than
+    one strip, with partial strips, and on every map-factor / filter path."""
     jx, iy, kz = grid
+    over = FUSION_OPTS[opts]
+    if over and grid != (29, 13, 7):
+        pytest.skip("option variants run on one grid")
     c = small_config2(jx, iy, kz) if jx > 10 else dm.case_config(0, jx=jx, iy=iy, kz=kz, hill_hw=1.0,
                                                                  noise_levels=2)
     hs = dm.build_case(c)
-    a = dm.run_case(hs, dm.case_params(c, precision=prec, unfused=1), 2)
-    b = dm.run_case(hs, dm.case_params(c, precision=prec, unfused=0), 2)
+    print(f"fused vs unfused {grid} prec={prec} {over}", flush=True)
+    a = dm.run_case(hs, dm.case_params(c, precision=prec, unfused=1, **over), 2)
+    b = dm.run_case(hs, dm.case_params(c, precision=prec, unfused=0, **over), 2)
     for n in PROGNOSTIC:
         np.testing.assert_array_equal(dm.interior(a[n]), dm.interior(b[n]), err_msg=n)
 
