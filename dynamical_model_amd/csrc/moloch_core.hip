@@ -771,18 +771,34 @@ __global__ void __launch_bounds__(256) k_wafv(Dev<T> D, Par P, FieldSet<Q> F, do
   // Pw levels m-3..m (slot 3 = m); Ww first-pass levels m-5..m-2
   Q S[5], ZU[5], ZD[5], ffz;
   Q Pw[NF][4], Ww[NF][4], F1[NF], F2[NF];
-  for (int m = 1; m <= nk + 4; m++) {
-    const int mm = m <= nk ? m : nk;
+  // the loads of iteration m (s at interface m, fmz and the fields at level
+  // min(m, kz), fmzf below it), issued one iteration ahead; past the column
+  // they repeat the last level and are not used
+  Q n_s, n_fz, n_ff, n_p[NF];
+  auto fetch = [&](int m) {
+    const int mm = m <= nk ? m : nk, ms = m <= nk + 1 ? m : nk + 1;
     const long xm = c0 + (long)(mm - 1) * pl;
-    const Q sm = m <= nk + 1 ? (Q)D.s[c0 + (long)(m - 1) * pl] : Q(0);
-    const Q fzm = (Q)D.fmz[xm];
+    n_s = (Q)D.s[c0 + (long)(ms - 1) * pl];
+    n_fz = (Q)D.fmz[xm];
+    n_ff = (Q)D.fmzf[xm + pl];
+#pragma unroll
+    for (int f = 0; f < NF; f++) n_p[f] = F.in[f][xm];
+  };
+  fetch(1);
+  for (int m = 1; m <= nk + 4; m++) {
+    const Q sm = m <= nk + 1 ? n_s : Q(0);
+    const Q fzm = n_fz, ff1 = n_ff;
+    Q pin[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) pin[f] = n_p[f];
+    fetch(m + 1);
     if (m == 1) {
       ffz = (Q)D.fmzf[c0];
 #pragma unroll
       for (int q = 0; q < 5; q++) S[q] = Q(0), ZU[q] = Q(0), ZD[q] = Q(0);
 #pragma unroll
       for (int f = 0; f < NF; f++) {
-        Q p1 = F.in[f][xm];
+        Q p1 = pin[f];
 #pragma unroll
         for (int q = 0; q < 4; q++) Pw[f][q] = p1;
         F1[f] = Q(0);
@@ -793,7 +809,6 @@ __global__ void __launch_bounds__(256) k_wafv(Dev<T> D, Par P, FieldSet<Q> F, do
     for (int q = 0; q < 4; q++) S[q] = S[q + 1], ZU[q] = ZU[q + 1], ZD[q] = ZD[q + 1];
     S[4] = sm;
     if (m <= nk) {
-      Q ff1 = (Q)D.fmzf[xm + pl];
       ZU[4] = dtrdz * fzm / ffz;  // zrfmu (:1010)
       ZD[4] = dtrdz * fzm / ff1;  // zrfmd (:1011)
       ffz = ff1;
@@ -801,7 +816,7 @@ __global__ void __launch_bounds__(256) k_wafv(Dev<T> D, Par P, FieldSet<Q> F, do
     const int kb = m - 2, kc = m - 4;
 #pragma unroll
     for (int f = 0; f < NF; f++) {
-      Q pm = F.in[f][xm];
+      Q pm = pin[f];
 #pragma unroll
       for (int q = 0; q < 3; q++) Pw[f][q] = Pw[f][q + 1];
       Pw[f][3] = pm;
