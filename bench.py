@@ -170,6 +170,7 @@ def main() -> int:
     ap.add_argument("--train-iters", type=int, default=6000, help="Adam steps of the corrector")
     ap.add_argument("--train-batch", type=int, default=1024, help="rows per Adam step")
     ap.add_argument("--samples", type=int, default=8192, help="training columns sampled per acoustic phase")
+    ap.add_argument("--chunk", type=int, default=65536, help="rows per corrector inference launch (multiple of 256)")
     ap.add_argument("--variants", default="fp64,fp32,learned")
     ap.add_argument("--check-tol", type=float, default=1.0,
                     help="corrector check: fall back when a field's error ratio exceeds this")
@@ -189,7 +190,7 @@ def main() -> int:
 
     learned_cfg = {
         "config": dict(nhid=512, samples_per_phase=args.samples,
-                       capacity=max(131072, 3 * args.lead * args.samples), chunk_rows=65536, check_every=50,
+                       capacity=max(131072, 3 * args.lead * args.samples), chunk_rows=args.chunk, check_every=50,
                        check_tol=args.check_tol, seed=11, engine=1),
         "batch": args.train_batch,
         "schedule": [(args.train_iters // 2, 1e-3), (args.train_iters // 4, 3e-4), (args.train_iters // 4, 1e-4)],

@@ -1249,18 +1249,17 @@ __global__ void __launch_bounds__(256) k_lac_features(Dev<T> D, LacSnap<T> S, Fe
   }
 }
 
-// Block tile of 256 rows x 256 outputs on v_mfma_f32_32x32x16_bf16: eight
-// waves (two per SIMD, so one wave's MFMAs cover the other's memory waits)
-// of 128 rows x 64 outputs each.  B comes from weight fragments packed in
-// the MFMA lane order, loaded DB k-steps ahead into registers and staged
-// through LDS one k-step (16) at a time.  A is either
-//  * the feature planes (layer 1): the block's 256 rows are one 64 x 4 tile
-//    of columns; for each pair of fl groups (16 fl, 9 stencil positions = 9
-//    k-steps) the 66 x 6 halo of the tile is copied into LDS as [column][16
-//    fl] and every stencil position reads it at its offset; the next pair is
-//    loaded when the current one starts;
-//  * row-major bf16 activations (layers 2 and 3), loaded DB k-steps ahead.
-// Epilogue: bias, then ReLU into bf16 H, or plain fp32 into Y.
+// Layer 1: block tile of 256 rows x 256 outputs on
+// v_mfma_f32_32x32x16_bf16: eight waves (two per SIMD, so one wave's MFMAs
+// cover the other's memory waits) of 128 rows x 64 outputs each.  B comes
+// from weight fragments packed in the MFMA lane order, loaded PD k-steps
+// ahead into registers and staged through LDS one k-step (16) at a time.  A
+// is the feature planes: the block's 256 rows are one 64 x 4 tile of
+// columns; for each pair of fl groups (16 fl, 9 stencil positions = 9
+// k-steps) the 66 x 6 halo of the tile is copied into LDS as [column][16 fl]
+// and every stencil position reads it at its offset; the next pair is loaded
+// when the current one starts.  Epilogue: bias, then ReLU into bf16 H (or
+// plain fp32 into Y).
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 lac_bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float lac_f32x16;
 
@@ -1288,17 +1287,14 @@ struct FusedArgs {
   FeatPlanes fp;         // layer 1 inputs
 };
 
-template <bool IMPLICIT>
 __global__ void __launch_bounds__(FT) k_mlp_fused(LacRows R, FusedArgs a) {
-  // The k loop is unrolled in groups of KG steps (9 = one pair of fl groups
-  // for the feature planes, 4 for activations) with prefetch depth PD
-  // dividing KG, so register slots are compile-time and the loop body is
-  // free of branches: prefetches past the end are clamped to the last stage
-  // and land in buffers nobody reads any more.  Threads without a halo
-  // column copy into spare LDS columns.
-  constexpr int KG = IMPLICIT ? 9 : 4, PD = IMPLICIT ? 3 : 4;
-  constexpr int ACOLS = IMPLICIT ? FT : FM;  // LDS columns (halo + spare) or rows
-  __shared__ __attribute__((aligned(16))) __bf16 sA[2][ACOLS * ACOL];
+  // The k loop is unrolled in groups of KG = 9 steps (one pair of fl
+  // groups) with prefetch depth PD dividing KG, so register slots are
+  // compile-time and the loop body is free of branches: prefetches past the
+  // end are clamped to the last stage and land in buffers nobody reads any
+  // more.  Threads without a halo column copy into spare LDS columns.
+  constexpr int KG = 9, PD = 3;
+  __shared__ __attribute__((aligned(16))) __bf16 sA[2][FT * ACOL];
   __shared__ __attribute__((aligned(16))) lac_bf16x8 sB[2][FN / 32][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 2, wn = wave & 3;  // rows wm*128, outputs wn*64
@@ -1323,27 +1319,16 @@ __global__ void __launch_bounds__(FT) k_mlp_fused(LacRows R, FusedArgs a) {
 
   // B of k-step ks: 8 fragments x 64 lanes, one per thread
   const lac_bf16x8 *bsrc = a.Bp + ((long)(n0 >> 5) + (tid >> 6)) * 64 + lane;
-  lac_bf16x8 breg[PD], areg[PD];
+  lac_bf16x8 breg[PD];
   auto loadB = [&](int ks, int slot) {
     ks = ks < nks ? ks : nks - 1;
     breg[slot] = bsrc[(long)ks * a.bnt * 64];
   };
   auto commitB = [&](int slot, int buf) { sB[buf][tid >> 6][lane] = breg[slot]; };
-  // row-major A of k-step ks: row tid/2, 8 k
-  const __bf16 *asrc = IMPLICIT ? nullptr : a.A + (rowb - a.row0 + (tid >> 1)) * a.lda + (tid & 1) * 8;
-  auto loadA = [&](int ks, int slot) {
-    if (!IMPLICIT) {
-      ks = ks < nks ? ks : nks - 1;
-      areg[slot] = *reinterpret_cast<const lac_bf16x8 *>(asrc + ks * 16);
-    }
-  };
-  auto commitA = [&](int slot, int buf) {
-    if (!IMPLICIT) *reinterpret_cast<lac_bf16x8 *>(&sA[buf][(tid >> 1) * ACOL + (tid & 1) * 8]) = areg[slot];
-  };
   // feature-plane halo: column tid of the tile's 66 x 6 halo (threads past
   // the halo re-read its last column into their spare LDS column)
-  const __bf16 *hsrc = nullptr;
-  if (IMPLICIT) {
+  const __bf16 *hsrc;
+  {
     const int t = (int)(rowb >> 8), c = tid < HALO_C ? tid : HALO_C - 1;
     const int tj = t % R.ntj, ti = t / R.ntj;
     hsrc = a.fp.F + ((long)(ti * 4 + c / HALO_J) * a.fp.fpj + tj * 64 + c % HALO_J) * 16;
@@ -1367,13 +1352,9 @@ __global__ void __launch_bounds__(FT) k_mlp_fused(LacRows R, FusedArgs a) {
 #pragma unroll
     for (int x = 0; x < 4; x++) {
       const int m = wm * 128 + 32 * x + (lane & 31);
-      if (IMPLICIT) {
-        const int cc = 2 * t + (lane >> 5), h = cc / 9, p = cc % 9;
-        const int col = ((m >> 6) + p / 3) * HALO_J + (m & 63) + p % 3;
-        av[x] = *reinterpret_cast<const lac_bf16x8 *>(&sA[abuf][col * ACOL + h * 8]);
-      } else {
-        av[x] = *reinterpret_cast<const lac_bf16x8 *>(&sA[abuf][m * ACOL + (lane >> 5) * 8]);
-      }
+      const int cc = 2 * t + (lane >> 5), h = cc / 9, p = cc % 9;
+      const int col = ((m >> 6) + p / 3) * HALO_J + (m & 63) + p % 3;
+      av[x] = *reinterpret_cast<const lac_bf16x8 *>(&sA[abuf][col * ACOL + h * 8]);
     }
 #pragma unroll
     for (int y = 0; y < 2; y++) bv[y] = sB[bbuf][wn * 2 + y][lane];
@@ -1384,34 +1365,25 @@ __global__ void __launch_bounds__(FT) k_mlp_fused(LacRows R, FusedArgs a) {
   };
 
   // prologue: stage 0 into LDS; stages 1 .. PD into register slots s % PD
-  if (IMPLICIT) {
-    loadH(0);
-    commitH(0);
-  }
+  loadH(0);
+  commitH(0);
   loadB(0, 0);
-  loadA(0, 0);
   commitB(0, 0);
-  commitA(0, 0);
 #pragma unroll
-  for (int u = 1; u <= PD; u++) {
-    loadB(u, u % PD);
-    loadA(u, u % PD);
-  }
+  for (int u = 1; u <= PD; u++) loadB(u, u % PD);
   __syncthreads();
   for (int k0 = 0; k0 < nks; k0 += KG) {
-    const int dg = k0 / 9;  // feature planes: the pair of fl groups of this group of k-steps
-    if (IMPLICIT) loadH(dg + 1);
+    const int dg = k0 / 9;  // the pair of fl groups of this group of k-steps
+    loadH(dg + 1);
 #pragma unroll
     for (int u = 0; u < KG; u++) {
       const int ks = k0 + u;
-      mfma_step(IMPLICIT ? (dg & 1) : (ks & 1), ks & 1, u);
+      mfma_step(dg & 1, ks & 1, u);
       // stage ks+1 from its register slot into the free LDS buffer, then
       // refill that slot with stage ks+1+PD (stage s lives in slot s % PD)
       commitB((u + 1) % PD, (ks + 1) & 1);
-      commitA((u + 1) % PD, (ks + 1) & 1);
-      if (IMPLICIT && u == KG - 1) commitH((dg + 1) & 1);
+      if (u == KG - 1) commitH((dg + 1) & 1);
       loadB(ks + 1 + PD, (u + 1) % PD);
-      loadA(ks + 1 + PD, (u + 1) % PD);
       __syncthreads();
     }
   }
@@ -1428,6 +1400,155 @@ __global__ void __launch_bounds__(FT) k_mlp_fused(LacRows R, FusedArgs a) {
       for (int r = 0; r < 16; r++) {
         const int m = wm * 128 + 32 * x + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         const long row = rowb - a.row0 + m;
+        const float v = acc[x][y][r] + bn;
+        if (a.H)
+          a.H[row * a.ldh + n] = (__bf16)(v > 0.f ? v : 0.f);
+        else
+          a.Y[row * a.ldy + n] = v;
+      }
+    }
+}
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// One 16-byte-per-lane global -> LDS copy: lane l's 16 bytes at gptr land at
+// LDS byte address lds + 16 l (lds wave-uniform).  Issued from inline asm so
+// that the compiler's wait insertion, which cannot tell the buffers apart,
+// adds no wait for it; the kernels wait with wait_vm before reading the
+// buffer, and before they end.  No output registers.
+__device__ __forceinline__ void dma16(const void *gptr, const void *lds_ptr) {
+  const unsigned lds = __builtin_amdgcn_readfirstlane((unsigned)(unsigned long)(const lds_void *)lds_ptr);
+  __asm__ volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(gptr), "s"(lds)
+                   : "memory", "m0");
+}
+
+// wait until at most n vector-memory loads of this wave are outstanding
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// Layers 2 and 3: row-major bf16 activations A (rows of this launch) times
+// packed weight fragments, the block tile and wave layout of k_mlp_fused.
+// Stages of two k-steps (32 k) go straight from global memory into one of
+// four LDS buffers (global_load_lds, 16 bytes per lane, lane l of a wave
+// instruction landing at base + 16 l), three stages ahead of the MFMAs, so
+// no registers hold prefetched data and a block barrier comes every two
+// k-steps.  An A row of a stage is 64 bytes, its four 16-byte chunks stored
+// at chunk ^ ((row >> 1) & 3) so the 16-byte reads of eight consecutive rows
+// hit distinct banks.  The MFMAs are issued in k_mlp_fused's order, so the
+// results are bit-identical.
+constexpr int DKS = 2, DNB = 4;  // k-steps per stage, LDS buffers
+__global__ void __launch_bounds__(FT) k_mlp_dense(FusedArgs a) {
+  // one LDS object per buffer and the stage loop unrolled over the buffers,
+  // so each DMA and each read names its buffer at compile time
+  __shared__ __attribute__((aligned(16))) __bf16 sA0[FM * DKS * 16], sA1[FM * DKS * 16], sA2[FM * DKS * 16],
+      sA3[FM * DKS * 16];
+  __shared__ __attribute__((aligned(16))) lac_bf16x8 sB0[DKS][FN / 32][64], sB1[DKS][FN / 32][64],
+      sB2[DKS][FN / 32][64], sB3[DKS][FN / 32][64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 2, wn = wave & 3;  // rows wm*128, outputs wn*64
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int lin = (b & 7) * (nb >> 3) + (b >> 3);  // consecutive items on one XCD
+  const int rt = lin / a.nN, ntile = lin % a.nN;
+  if (rt >= a.nrt) return;  // the whole block
+  const long rowb = (long)rt * FM;  // within the launch
+  const int n0 = ntile * FN;
+  const int nst = a.nks / DKS;  // nks: a multiple of DKS (K a multiple of 32)
+
+  lac_f32x16 acc[4][2];
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[x][y][r] = 0.f;
+
+  // this lane's share of a stage: A rows g*16 + lane/4 of row groups g =
+  // 2*wave + q, chunk slot lane%4; B fragment lane of (k-step, n-tile) pairs
+  // p = 2*wave + q
+  const __bf16 *asrc[2];
+  const lac_bf16x8 *bsrc[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int row = (2 * wave + q) * 16 + (lane >> 2);
+    const int c = (lane & 3) ^ ((row >> 1) & 3);
+    asrc[q] = a.A + (rowb + row) * a.lda + c * 8;
+    const int pr = 2 * wave + q, t = pr >> 3, fr = pr & 7;
+    bsrc[q] = a.Bp + ((long)t * a.bnt + (n0 >> 5) + fr) * 64 + lane;
+  }
+  auto issue = [&](int st, __bf16 *dA, lac_bf16x8 (*dB)[FN / 32][64]) {
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      dma16(asrc[q] + st * (DKS * 16), &dA[(2 * wave + q) * 16 * (DKS * 16)]);
+      const int pr = 2 * wave + q;
+      dma16(bsrc[q] + (long)st * DKS * a.bnt * 64, &dB[pr >> 3][pr & 7][0]);
+    }
+  };
+  auto compute = [&](const __bf16 *cA, const lac_bf16x8 (*cB)[FN / 32][64]) {
+#pragma unroll
+    for (int t = 0; t < DKS; t++) {
+      lac_bf16x8 av[4], bv[2];
+#pragma unroll
+      for (int x = 0; x < 4; x++) {
+        const int m = wm * 128 + 32 * x + (lane & 31);
+        const int c = (2 * t + (lane >> 5)) ^ ((m >> 1) & 3);
+        av[x] = *reinterpret_cast<const lac_bf16x8 *>(&cA[m * (DKS * 16) + c * 8]);
+      }
+#pragma unroll
+      for (int y = 0; y < 2; y++) bv[y] = cB[t][wn * 2 + y][lane];
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++)
+          acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[x], bv[y], acc[x][y], 0, 0, 0);
+    }
+  };
+  // stage st from its buffer; its loads have landed when at most the later
+  // stages' (four per stage) are outstanding; stage st+3 then goes to the
+  // buffer of stage st-1, which every wave finished reading before the
+  // barrier
+  auto stage = [&](int st, const __bf16 *cA, const lac_bf16x8 (*cB)[FN / 32][64], __bf16 *nA,
+                   lac_bf16x8 (*nB)[FN / 32][64]) {
+    if (st + 2 < nst)
+      wait_vm<8>();
+    else if (st + 1 < nst)
+      wait_vm<4>();
+    else
+      wait_vm<0>();
+    // a bare barrier: the explicit wait above already covers this wave's
+    // DMA, and a workgroup fence would wait for the later stages' too
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+    compute(cA, cB);
+    if (st + 3 < nst) issue(st + 3, nA, nB);
+  };
+
+  // only stages below nst are loaded or computed (K may be one or a few stages)
+  issue(0, sA0, sB0);
+  if (nst > 1) issue(1, sA1, sB1);
+  if (nst > 2) issue(2, sA2, sB2);
+  for (int st = 0; st < nst; st += DNB) {
+    stage(st, sA0, sB0, sA3, sB3);
+    if (st + 1 < nst) stage(st + 1, sA1, sB1, sA0, sB0);
+    if (st + 2 < nst) stage(st + 2, sA2, sB2, sA1, sB1);
+    if (st + 3 < nst) stage(st + 3, sA3, sB3, sA2, sB2);
+  }
+
+  // epilogue: D[m][n] of tile (x, y): n = lane%32, m = 8*(r/4) + 4*(lane/32) + r%4
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) {
+      const int n = n0 + wn * 64 + 32 * y + (lane & 31);
+      if (n >= a.N) continue;
+      const float bn = a.bias[n];
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int m = wm * 128 + 32 * x + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const long row = rowb + m;
         const float v = acc[x][y][r] + bn;
         if (a.H)
           a.H[row * a.ldh + n] = (__bf16)(v > 0.f ? v : 0.f);
@@ -2553,7 +2674,7 @@ struct Core : CoreBase {
     a.bias = p->bvec[0];
     a.H = p->H1;
     a.ldh = p->nhid;
-    hipLaunchKernelGGL(k_mlp_fused<true>, grid(a.nN), dim3(FT), 0, stream, L.rows, a);
+    hipLaunchKernelGGL(k_mlp_fused, grid(a.nN), dim3(FT), 0, stream, L.rows, a);
     // layer 2
     a.Bp = reinterpret_cast<const lac_bf16x8 *>(p->Wp[1]);
     a.bnt = p->wp_nt[1];
@@ -2562,7 +2683,7 @@ struct Core : CoreBase {
     a.A = p->H1;
     a.lda = p->nhid;
     a.H = p->H2;
-    hipLaunchKernelGGL(k_mlp_fused<false>, grid(a.nN), dim3(FT), 0, stream, L.rows, a);
+    hipLaunchKernelGGL(k_mlp_dense, grid(a.nN), dim3(FT), 0, stream, a);
     // layer 3
     a.Bp = reinterpret_cast<const lac_bf16x8 *>(p->Wp[2]);
     a.bnt = p->wp_nt[2];
@@ -2574,7 +2695,7 @@ struct Core : CoreBase {
     a.H = nullptr;
     a.Y = L.Y + r0 * L.nout;
     a.ldy = L.nout;
-    hipLaunchKernelGGL(k_mlp_fused<false>, grid(a.nN), dim3(FT), 0, stream, L.rows, a);
+    hipLaunchKernelGGL(k_mlp_dense, grid(a.nN), dim3(FT), 0, stream, a);
   }
 
   void lac_apply() {
