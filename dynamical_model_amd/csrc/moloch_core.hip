@@ -952,6 +952,36 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, do
       zs[f] = mflux(mz, ma, w[f][0], w[f][1], w[f][2], w[f][3]);
     }
   }
+  // field-independent loads of the row centred at ic, issued a row ahead:
+  // v, fmz and rmv at its north neighbour (the south ones are carried from
+  // the previous row), fmz east and west, u here and east, the map factors
+  struct RowIn {
+    Q vn, fn, fe, fw, uc, up, rvn, m0, m1, m2;
+  };
+  auto row_in = [&](int ic, int in) {
+    RowIn o;
+    o.vn = (Q)D.v[at3(jw, in)];
+    o.fn = (Q)D.fmz[at3(jw, in)];
+    o.fe = (Q)D.fmz[at3(jp, ic)];
+    o.fw = (Q)D.fmz[at3(jm, ic)];
+    o.uc = (Q)D.u[at3(jw, ic)];
+    o.up = (Q)D.u[at3(jp, ic)];
+    o.rvn = (Q)D.rmv[g.at2(jw, in)];
+    if constexpr (ROT) {
+      o.m0 = (Q)D.mx[g.at2(jw, ic)];
+      o.m1 = (Q)D.mu[g.at2(jw, ic)];
+      o.m2 = Q(0);
+    } else {
+      o.m0 = (Q)D.mx2[g.at2(jw, ic)];
+      o.m1 = (Q)D.rmu[g.at2(jw, ic)];
+      o.m2 = (Q)D.rmu[g.at2(jp, ic)];
+    }
+    return o;
+  };
+  // carried south values of the current row
+  Q v_c = (Q)D.v[at3(jw, i_c)], rv_c = (Q)D.rmv[g.at2(jw, i_c)];
+  Q f_s = (Q)D.fmz[at3(jw, i_m1)], f_c = (Q)D.fmz[at3(jw, i_c)];
+  RowIn nx = row_in(i_c, i_p1);
   // loads of the first row
   Q nw[NF], np[NF];
 #pragma unroll
@@ -966,6 +996,7 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, do
       w[f][4] = nw[f];
       pp[f] = np[f];
     }
+    const RowIn ci = nx;
     if (r + 1 < nrows) {  // uniform over the wave
       const int i_n = nxt(i_p2);
 #pragma unroll
@@ -973,18 +1004,25 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, do
         nw[f] = F.wz[f][at3(jw, i_n)];
         np[f] = F.in[f][at3(jw, i_p1)];
       }
+      nx = row_in(i_p1, i_p2);
     }
     // field-independent coefficients of row i_c
     Q mzn, man;  // north face (south face of row i_p1)
-    mface(i_p1, mzn, man);
+    if constexpr (ROT) {
+      mzn = ci.vn * dtrdy;
+      man = Q(0.5) * ci.vn;
+    } else {
+      mzn = ci.vn * ci.rvn * dtrdy;
+      man = Q(0.5) * ci.vn * ci.rvn;
+    }
     Q pfs, pfn, pcd, pm2;
     {
-      const Q f0 = (Q)D.fmz[at3(jw, i_c)], fn = (Q)D.fmz[at3(jw, i_p1)], fs = (Q)D.fmz[at3(jw, i_m1)];
-      const Q vc = (Q)D.v[at3(jw, i_c)], vp = (Q)D.v[at3(jw, i_p1)];
+      const Q f0 = f_c, fn = ci.fn, fs = f_s;
+      const Q vc = v_c, vp = ci.vn;
       if constexpr (ROT) {
-        const Q mx = (Q)D.mx[g.at2(jw, i_c)];
-        const Q zhxvtn = dtrdy * (Q)D.rmv[g.at2(jw, i_p1)] * mx;
-        const Q zhxvts = dtrdy * (Q)D.rmv[g.at2(jw, i_c)] * mx;
+        const Q mx = ci.m0;
+        const Q zhxvtn = dtrdy * ci.rvn * mx;
+        const Q zhxvts = dtrdy * rv_c * mx;
         pfn = zhxvtn * Q(2) * f0 / (f0 + fn);
         pfs = zhxvts * Q(2) * f0 / (f0 + fs);
         pcd = vp * pfn - vc * pfs;
@@ -992,16 +1030,16 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, do
       } else {
         pfn = dtrdy * Q(2) * f0 / (f0 + fn);
         pfs = dtrdy * Q(2) * f0 / (f0 + fs);
-        pcd = vp * (Q)D.rmv[g.at2(jw, i_p1)] * pfn - vc * (Q)D.rmv[g.at2(jw, i_c)] * pfs;
-        pm2 = (Q)D.mx2[g.at2(jw, i_c)];
+        pcd = vp * ci.rvn * pfn - vc * rv_c * pfs;
+        pm2 = ci.m0;
       }
     }
     Q zz, za, zfw, zfe, zcd, zm2;
     {
-      const Q uc = (Q)D.u[at3(jw, i_c)], up = (Q)D.u[at3(jp, i_c)];
-      const Q f0 = (Q)D.fmz[at3(jw, i_c)], fe = (Q)D.fmz[at3(jp, i_c)], fw = (Q)D.fmz[at3(jm, i_c)];
+      const Q uc = ci.uc, up = ci.up;
+      const Q f0 = f_c, fe = ci.fe, fw = ci.fw;
       if constexpr (ROT) {
-        const Q mu = (Q)D.mu[g.at2(jw, i_c)];
+        const Q mu = ci.m1;
         zz = uc * mu * dtrdx;
         za = Q(0.5) * uc;
         const Q zcostx = dtrdx * mu;
@@ -1010,15 +1048,20 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, do
         zcd = up * zfe - uc * zfw;
         zm2 = Q(1);
       } else {
-        const Q rm = (Q)D.rmu[g.at2(jw, i_c)];
+        const Q rm = ci.m1;
         zz = uc * rm * dtrdx;
         za = Q(0.5) * uc * rm;
         zfe = dtrdx * Q(2) * f0 / (f0 + fe);
         zfw = dtrdx * Q(2) * f0 / (f0 + fw);
-        zcd = up * (Q)D.rmu[g.at2(jp, i_c)] * zfe - uc * rm * zfw;
-        zm2 = (Q)D.mx2[g.at2(jw, i_c)];
+        zcd = up * ci.m2 * zfe - uc * rm * zfw;
+        zm2 = ci.m0;
       }
     }
+    // the north values become the next row's south ones
+    v_c = ci.vn;
+    rv_c = ci.rvn;
+    f_s = f_c;
+    f_c = ci.fn;
     const long xo = at3(ja <= nj ? ja : jw, i_c);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
