@@ -65,6 +65,9 @@ extern "C" const char *mlc_last_error(void) { return g_errbuf; }
 namespace mlc {
 
 constexpr int JOFF = 31;
+// segmented column solve: level segments per column, levels per segment,
+// columns per block
+constexpr int SEG_NS = 8, SEG_L = 8, SEG_CW = 32;
 
 struct Geo {
   int nj, ni, nk;
@@ -290,7 +293,8 @@ __global__ void __launch_bounds__(256) k_sound_a(Dev<T> D, Par P, double dts) {
 // ======================================================== sound, phase B
 // divdamp (:853-895), vertical divergence (:627-630), the implicit w solve
 // (:634-686), the new Exner function and theta (:690-728), one column per
-// lane.  Forward elimination runs bottom-up storing the elimination
+// lane: the reference's loop order, kept for params.unfused and for kz
+// beyond what k_sound_bs holds in registers.  Forward elimination runs bottom-up storing the elimination
 // coefficients in FP64 scratch; back substitution runs top-down and
 // finishes each level as soon as w above and below it are known.
 // split != 0 (divergence filter on): stop after theta, leaving zdiv2 in zds.
@@ -406,6 +410,200 @@ __global__ void __launch_bounds__(256) k_sound_b(Dev<T> D, Par P, double dts, in
   finish(nk, (double)wprev_t, (double)D.w[g.at(j, i, nk + 1)]);
 }
 
+// ============================================ sound, phase B (segmented)
+// The arithmetic of k_sound_b without divdamp's wind update, which
+// k_sound_c<T, true> applies as it reads the winds.  Each column's unknown
+// levels 2..kz are cut into NS contiguous segments of at most L levels
+// (segment 0, the top one, also takes level 1); a block holds CW columns of
+// one row and one lane per (column, segment), 64 / CW segments per wave.
+// Each lane keeps what the column solve and the theta/pi update need for its
+// levels in registers: the explicit right-hand side and the two coupling
+// coefficients (FP64), the divergence, pi, and with KEEP also theta and the
+// old w.  The forward elimination (bottom segment first) and the back
+// substitution (top segment first) pass through the segments in turn,
+// handing the (g, c) pair and then w across segment boundaries through LDS,
+// one block barrier per segment: the operations and their order are those of
+// the one-lane sweep of k_sound_b, so the results are bit-identical, while
+// neither the elimination coefficients nor the divergence go through HBM.
+// Needs 2 <= kz <= NS*L + 1; nbj = number of CW-column tiles per row.
+template <typename T, int NS, int L, int CW, bool KEEP>
+__global__ void __launch_bounds__(NS * CW) k_sound_bs(Dev<T> D, Par P, double dts, int ns, int split, int nbj) {
+  static_assert(64 % CW == 0 && (NS * CW) % 64 == 0, "whole waves of whole columns");
+  __shared__ double hg[NS][CW], hc[NS][CW], hw[NS][CW], hw1[NS][CW];
+  const Geo &g = D.g;
+  const int lane = threadIdx.x % CW;                           // column within the block
+  const int wv = threadIdx.y * (64 / CW) + threadIdx.x / CW;  // segment
+  // Workgroups are dispatched to the 8 XCDs round robin; number the tiles so
+  // that each XCD sweeps a contiguous band of rows (the i +- 1 divergence
+  // reads then meet rows its own L2 has seen).
+  const int nblk = gridDim.x, lin = blockIdx.x;
+  const int per = nblk >> 3, rem = nblk & 7, xcd = lin & 7;
+  const int tile = (xcd < rem ? xcd * (per + 1) : rem * (per + 1) + (xcd - rem) * per) + (lin >> 3);
+  const int i = tile / nbj + 1;
+  const int jr = (tile % nbj) * CW + lane + 1;
+  const bool act = jr <= g.nj;
+  const int j = act ? jr : g.nj;  // idle lanes read a valid column and store nothing
+  const int nk = g.nk;
+  const int nu = nk - 1, base = nu / NS, rm = nu % NS;
+  const int cnt = base + (wv < rm ? 1 : 0);            // empty segments only at the bottom
+  const int a = 2 + wv * base + (wv < rm ? wv : rm);  // levels a .. a+cnt-1
+  const int bl = a + cnt - 1;
+  const double dtrdz = dts / P.dzita;
+  const double zcs2 = dtrdz * dtrdz * kRdrcv;
+  const T ddamp2 = (T)(kXdamp * 0.015625 / dts);
+  const T dtrdz_t = (T)dtrdz;
+  const long surf = g.at(j, i, nk + 1);
+
+  // smoothed + vertical divergence of the level at x (level_a of k_sound_b)
+  auto level_z = [&](long x) -> T {
+    T zc = D.zdh[x];
+    T zjm = D.zdh[x - 1], zjp = D.zdh[x + 1], zim = D.zdh[x - g.pj], zip = D.zdh[x + g.pj];
+    T p2d = T(0.125) * (zjm + zjp + zim + zip) - T(0.5) * zc;
+    T z = zc + ddamp2 * p2d;
+    z = z + D.fmz[x] * dtrdz_t * (D.s[x] - D.s[x + g.plane]);
+    return z;
+  };
+
+  double rz[L], ru[L], rd[L];  // zwexpl, zu, zd; then g, c; then the new w
+  T zz[L], wo[L], th[L];
+  double pp[L];
+  // level a-1 (the segment's upper neighbour; level 1 for wave 0)
+  T zt = 0, tht = 0, fzt = 0;
+  double ppt = 0;
+  if (cnt > 0) {
+    {
+      const long x = g.at(j, i, a - 1);
+      zt = level_z(x);
+      tht = D.tetav[x];
+      ppt = D.pai[x];
+      fzt = D.fmz[x];
+    }
+    double zkm = (double)zt, tkm = (double)tht, pkm = ppt, fzkm = (double)fzt;
+    double qskm = (double)D.qsat[g.at(j, i, a - 1)];
+#pragma unroll
+    for (int l = 0; l < L; l++) {
+      const int k = a + l <= bl ? a + l : bl;  // past the segment: repeat its last level
+      const long xk = g.at(j, i, k), xm = xk - g.plane;
+      const T zkt = level_z(xk);
+      zz[l] = zkt;
+      const T tkt = D.tetav[xk], wkt = D.w[xk];
+      if (KEEP) {
+        th[l] = tkt;
+        wo[l] = wkt;
+      }
+      pp[l] = D.pai[xk];
+      const double zk = (double)zkt, tk = (double)tkt, pk = pp[l], fzk = (double)D.fmz[xk];
+      const double qsk = (double)D.qsat[xk];
+      const double wk = (double)wkt;
+      const double ffk = (double)D.fmzf[xk];
+      double zrom1w = 0.5 * kCpd * ffk * (tkm + tk);
+      zrom1w = zrom1w - kCpd * wk * ffk * ffk * (double)ns * dtrdz * (tkm - tk);
+      if ((double)D.qv[xk] > 0.96 * qsk && wk > 0.1) {
+        double zqs = 0.5 * (qsk + qskm);
+        double tm = (double)D.t[xm];
+        double zdth = kEgrav * wk * (double)(ns - 1) * dts * kWlhv * kWlhv * zqs / (kCpd * pkm * kRwat * tm * tm);
+        zrom1w = zrom1w + zdth * ffk;
+      }
+      double zwexpl = wk - zrom1w * dtrdz * (pkm - pk) - kEgrav * dts;
+      zwexpl = zwexpl + kRdrcv * zrom1w * dtrdz * (pkm * zkm - pk * zk);
+      double ff = D.ffilt[k - 1];
+      rz[l] = zwexpl;
+      ru[l] = zcs2 * fzkm * zrom1w * pkm + ff;
+      rd[l] = zcs2 * fzk * zrom1w * pk + ff;
+      zkm = zk;
+      tkm = tk;
+      pkm = pk;
+      fzkm = fzk;
+      qskm = qsk;
+    }
+  }
+
+  // forward elimination, bottom segment first: g_k, c_k from g_{k+1}, c_{k+1}
+  for (int st = NS - 1; st >= 0; st--) {
+    if (wv == st) {
+      double g1, c1;
+      if (st == NS - 1) {
+        g1 = (double)D.w[surf];  // w(kz+1) from phase A
+        c1 = 0.0;                 // wwkw(kzp1) = 0
+      } else {
+        g1 = hg[st + 1][lane];
+        c1 = hc[st + 1][lane];
+      }
+#pragma unroll
+      for (int l = L - 1; l >= 0; l--) {
+        if (l < cnt) {
+          double zrapp = 1.0 / (1.0 + rd[l] + ru[l] - rd[l] * c1);
+          double gk = zrapp * (rz[l] + rd[l] * g1);
+          double ck = zrapp * ru[l];
+          rz[l] = gk;
+          ru[l] = ck;
+          g1 = gk;
+          c1 = ck;
+        }
+      }
+      hg[st][lane] = g1;
+      hc[st][lane] = c1;
+    }
+    __syncthreads();
+  }
+  // back substitution, top segment first
+  const T w1t = D.w[g.at(j, i, 1)];  // w(1) = 0
+  for (int st = 0; st < NS; st++) {
+    if (wv == st) {
+      double wp = st == 0 ? (double)w1t : hw[st - 1][lane];
+#pragma unroll
+      for (int l = 0; l < L; l++) {
+        if (l < cnt) {
+          double wn = rz[l] + ru[l] * wp;
+          rz[l] = wn;
+          wp = wn;
+        }
+      }
+      hw[st][lane] = wp;
+      hw1[st][lane] = cnt > 0 ? rz[0] : wp;
+    }
+    __syncthreads();
+  }
+  if (cnt == 0) return;
+
+  // w, deltaw; theta and pi of each level once w above and below are known
+  auto finish = [&](long x, int k, T zk, T fzk, T thk, double po, T wtop, T wbot) {
+    T zf = zk + dtrdz_t * fzk * (wtop - wbot);
+    T qv = D.qv[x], qc = D.qc[x];
+    if (P.fulleq) {
+      zf = zf * (T(1) + T(0.86) * qv + T(3.2) * qc) / (T(1) + T(0.96) * qv + T(4.8) * qc);
+      T tet = thk * (T(1) + (T)kRdrcv * zf * (T(0.25) * qv + T(4.2) * D.qwltot[x] + T(2.1) * D.qwitot[x]));
+      D.tetav[x] = tet;
+      store_faces(D.tetav, g, j, i, k, tet);
+    }
+    if (split) {
+      D.zds[x] = zf;
+      return;
+    }
+    if (!P.halo_mpi) store_faces(D.pai, g, j, i, k, po);
+    double pn = po * (1.0 - kRdrcv * (double)zf);
+    D.pai[x] = pn;
+    if (P.halo_mpi) store_faces(D.pai, g, j, i, k, pn);
+  };
+  if (!act) return;
+  const T wnext = bl == nk ? D.w[surf] : (T)hw1[wv + 1 < NS ? wv + 1 : wv][lane];
+  if (wv == 0) finish(g.at(j, i, 1), 1, zt, fzt, tht, ppt, w1t, (T)rz[0]);
+#pragma unroll
+  for (int l = 0; l < L; l++) {
+    if (l < cnt) {
+      const int k = a + l;
+      const long x = g.at(j, i, k);
+      const T wn_t = (T)rz[l];
+      T dw = -(KEEP ? wo[l] : D.w[x]) + wn_t;
+      D.w[x] = wn_t;
+      D.deltaw[x] = dw;
+      store_faces(D.deltaw, g, j, i, k, dw);
+      const T wb = l + 1 < cnt ? (T)rz[l + 1 < L ? l + 1 : l] : wnext;
+      finish(x, k, zz[l], D.fmz[x], KEEP ? th[l] : D.tetav[x], pp[l], wn_t, wb);
+    }
+  }
+}
+
 // Exner update after the divergence filter (split path): :716-728
 template <typename T>
 __global__ void __launch_bounds__(256) k_sound_b2(Dev<T> D, Par P) {
@@ -421,10 +619,14 @@ __global__ void __launch_bounds__(256) k_sound_b2(Dev<T> D, Par P) {
 }
 
 // ======================================================== sound, phase C
-// Horizontal momentum, Equations 17 and 18 (:730-838).  u and v hold the
-// post-divdamp interior and the substep-start halos, i.e. exactly the ud/vd
-// snapshots of :730-735; results go to un/vn.  Evaluated in FP64.
-template <typename T>
+// Horizontal momentum, Equations 17 and 18 (:730-838); results go to un/vn,
+// evaluated in FP64.  The winds read are the ud/vd snapshots of :730-735:
+// the post-divdamp interior and the substep-start halos.  DAMP = false: u
+// and v already hold them (k_sound_b applied divdamp).  DAMP = true: u and v
+// hold the substep-start winds and each interior value read gets divdamp's
+// increment (:865-872) here, from the horizontal divergence zdiv2 of phase
+// A, in the operand order of k_sound_b, so the winds agree bit for bit.
+template <typename T, bool DAMP>
 __global__ void __launch_bounds__(256) k_sound_c(Dev<T> D, Par P, double dts) {
   COL_IDX
   const double dtrdx = dts / P.dx, dtrdy = dts / P.dx;
@@ -434,6 +636,24 @@ __global__ void __launch_bounds__(256) k_sound_c(Dev<T> D, Par P, double dts) {
   const double hx = (double)D.hx[c2], hy = (double)D.hy[c2];
   const double coru = (double)D.coru[c2], corv = (double)D.corv[c2];
   const long pj = g.pj;
+  // divdamp of the winds read (DAMP): the points are u(j,i), u(j,i-1),
+  // u(j+1,i), u(j+1,i-1) and v(j,i), v(j-1,i), v(j-1,i+1), v(j,i+1); each
+  // gets xknu*ddamp*m*(zdiv2 difference) if it is an interior point.  Every
+  // load below lies within the two-point halo, so all are unconditional and
+  // the interior tests (column constants) only select.
+  const T ddamp = (T)(0.125 * (P.dx / dts));
+  const bool jlo = j >= 2, jhi = j <= g.nj - 1, ilo = i >= 2, ihi = i <= g.ni - 1;
+  T mu_c = 0, mu_im = 0, mu_jp = 0, mu_jpim = 0, mv_c = 0, mv_jm = 0, mv_jmip = 0, mv_ip = 0;
+  if (DAMP) {
+    mu_c = D.mu[c2];
+    mu_im = D.mu[c2 - pj];
+    mu_jp = D.mu[c2 + 1];
+    mu_jpim = D.mu[c2 + 1 - pj];
+    mv_c = D.mv[c2];
+    mv_jm = D.mv[c2 - 1];
+    mv_jmip = D.mv[c2 - 1 + pj];
+    mv_ip = D.mv[c2 + pj];
+  }
   long x = g.at(j, i, 1);
   double dwc = (double)D.deltaw[x], dwjm = (double)D.deltaw[x - 1], dwim = (double)D.deltaw[x - pj];
   for (int k = 1; k <= g.nk; k++, x += g.plane) {
@@ -442,22 +662,53 @@ __global__ void __launch_bounds__(256) k_sound_c(Dev<T> D, Par P, double dts) {
     double tc = (double)D.tetav[x];
     double pc = D.pai[x];
     double gzh = D.gzitakh[k - 1];
+    T u_c = D.u[x], u_im = D.u[x - pj], u_jp = D.u[x + 1], u_jpim = D.u[x + 1 - pj];
+    T v_c = D.v[x], v_jm = D.v[x - 1], v_jmip = D.v[x - 1 + pj], v_ip = D.v[x + pj];
+    if (DAMP) {
+      const T xk = (T)D.xknu[k - 1];
+      const T z00 = D.zdh[x], zm0 = D.zdh[x - 1], zp0 = D.zdh[x + 1];
+      const T z0m = D.zdh[x - pj], zmm = D.zdh[x - 1 - pj], zpm = D.zdh[x + 1 - pj];
+      const T z0p = D.zdh[x + pj], zmp = D.zdh[x - 1 + pj];
+      // the operand order of k_sound_b's level update
+      const T du_c = u_c + xk * ddamp * mu_c * (z00 - zm0);
+      const T du_im = u_im + xk * ddamp * mu_im * (z0m - zmm);
+      const T du_jp = u_jp + xk * ddamp * mu_jp * (zp0 - z00);
+      const T du_jpim = u_jpim + xk * ddamp * mu_jpim * (zpm - z0m);
+      T dv_c, dv_jm, dv_jmip, dv_ip;
+      if (P.rotllr) {
+        dv_c = v_c + xk * ddamp * (z00 - z0m);
+        dv_jm = v_jm + xk * ddamp * (zm0 - zmm);
+        dv_jmip = v_jmip + xk * ddamp * (zmp - zm0);
+        dv_ip = v_ip + xk * ddamp * (z0p - z00);
+      } else {
+        dv_c = v_c + xk * ddamp * mv_c * (z00 - z0m);
+        dv_jm = v_jm + xk * ddamp * mv_jm * (zm0 - zmm);
+        dv_jmip = v_jmip + xk * ddamp * mv_jmip * (zmp - zm0);
+        dv_ip = v_ip + xk * ddamp * mv_ip * (z0p - z00);
+      }
+      u_c = du_c;
+      u_im = ilo ? du_im : u_im;
+      u_jp = jhi ? du_jp : u_jp;
+      u_jpim = (jhi && ilo) ? du_jpim : u_jpim;
+      v_c = dv_c;
+      v_jm = jlo ? dv_jm : v_jm;
+      v_jmip = (jlo && ihi) ? dv_jmip : v_jmip;
+      v_ip = ihi ? dv_ip : v_ip;
+    }
     // Equation 17
     {
       double zfz = kEgrav * dts + 0.25 * (dwjm + dwjm1 + dwc + dwc1);
       double zrom1u = 0.5 * kCpd * ((double)D.tetav[x - 1] + tc);
-      double zcor1u = coru * dts * 0.25 *
-                      ((double)D.v[x] + (double)D.v[x - 1] + (double)D.v[x - 1 + pj] + (double)D.v[x + pj]);
-      double un = (double)D.u[x] + zcor1u - zfz * hx * gzh - zcx * zrom1u * (pc - D.pai[x - 1]);
+      double zcor1u = coru * dts * 0.25 * ((double)v_c + (double)v_jm + (double)v_jmip + (double)v_ip);
+      double un = (double)u_c + zcor1u - zfz * hx * gzh - zcx * zrom1u * (pc - D.pai[x - 1]);
       D.un[x] = (T)un;
     }
     // Equation 18
     {
       double zfz = kEgrav * dts + 0.25 * (dwim + dwim1 + dwc + dwc1);
       double zrom1v = 0.5 * kCpd * ((double)D.tetav[x - pj] + tc);
-      double zcor1v = corv * dts * 0.25 *
-                      ((double)D.u[x] + (double)D.u[x - pj] + (double)D.u[x + 1] + (double)D.u[x + 1 - pj]);
-      double vn = (double)D.v[x] - zcor1v - zfz * hy * gzh - zcy * zrom1v * (pc - D.pai[x - pj]);
+      double zcor1v = corv * dts * 0.25 * ((double)u_c + (double)u_im + (double)u_jp + (double)u_jpim);
+      double vn = (double)v_c - zcor1v - zfz * hy * gzh - zcy * zrom1v * (pc - D.pai[x - pj]);
       D.vn[x] = (T)vn;
     }
     dwc = dwc1;
@@ -2424,6 +2675,11 @@ struct Core : CoreBase {
     launch(KK_HALO, 0.0, k_halo<F>, dim3((n + 255) / 256, nlev), dim3(256), f, g, mode);
   }
 
+  // The segmented column solve (k_sound_bs, divdamp folded into k_sound_c)
+  // unless params.unfused asks for the one-lane sweep, or kz exceeds what
+  // SEG_NS segments of SEG_L levels hold.
+  bool segmented_solve() const { return !prm.unfused && g.nk >= 2 && g.nk <= SEG_NS * SEG_L + 1; }
+
   // One acoustic substep of length dts (the body of the loop at :511-839);
   // ns is the substep's ordinal, which enters the w solve (:641-657).
   void substep(double dts, int ns) {
@@ -2432,7 +2688,17 @@ struct Core : CoreBase {
     halo(D.v, g.nk, 1);
     launch(KK_SOUND_A, fbytes(3, 0), k_sound_a<T>, col_grid(), blk(), D, P, dts);
     int split = prm.divfilter ? 1 : 0;
-    launch(KK_SOUND_B, fbytes(13, 2), k_sound_b<T>, col_grid(), blk(), D, P, dts, ns, split);
+    const bool seg = segmented_solve();
+    if (seg) {
+      // FP32: theta and the old w are read again at the end rather than held
+      // (166 VGPRs, three waves per SIMD); FP64 holds them (two either way).
+      constexpr bool keep = sizeof(T) == sizeof(double);
+      const int nbj = (g.nj + SEG_CW - 1) / SEG_CW;
+      launch(KK_SOUND_B, fbytes(13, 2), k_sound_bs<T, SEG_NS, SEG_L, SEG_CW, keep>, dim3(nbj * g.ni),
+             dim3(64, SEG_NS * SEG_CW / 64), D, P, dts, ns, split, nbj);
+    } else {
+      launch(KK_SOUND_B, fbytes(13, 2), k_sound_b<T>, col_grid(), blk(), D, P, dts, ns, split);
+    }
     if (split) {
       // divergence_filter (:423-438) then the Exner update
       halo(D.zds, g.nk, 0);
@@ -2440,7 +2706,10 @@ struct Core : CoreBase {
              (const double *)D.xknu, prm.anu2);
       launch(KK_SOUND_B2, fbytes(1, 2), k_sound_b2<T>, col_grid(), blk(), D, P);
     }
-    launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T>, col_grid(), blk(), D, P, dts);
+    if (seg)
+      launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T, true>, col_grid(), blk(), D, P, dts);
+    else
+      launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T, false>, col_grid(), blk(), D, P, dts);
     std::swap(D.u, D.un);
     std::swap(D.v, D.vn);
   }

@@ -117,7 +117,9 @@ typedef struct mlc_params {
   int precision;    /* MLC_FP64 or MLC_FP32                               */
   int device;       /* HIP device ordinal                                 */
   int unfused;      /* 1: one kernel per phase as in the reference's loop */
-                    /* nests (for comparisons); 0: fused kernels (default) */
+                    /* nests and the one-lane column solve (for           */
+                    /* comparisons); 0: fused and segmented kernels       */
+                    /* (default); both give bit-identical results         */
 } mlc_params;
 
 /* lprint diagnostics of nompi/mod_moloch.F90:395-419 */

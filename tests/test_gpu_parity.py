@@ -174,6 +174,27 @@ def test_fused_equals_unfused(prec, grid, opts):
         np.testing.assert_array_equal(dm.interior(a[n]), dm.interior(b[n]), err_msg=n)
 
 
+@pytest.mark.parametrize("prec", [dm.MLC_FP64, dm.MLC_FP32])
+@pytest.mark.parametrize("kz", [3, 9, 17, 60, 65, 66])
+def test_segmented_solve_equals_one_lane(prec, kz):
+    """The segmented column solve (a column's levels split over eight lanes,
+    elimination coefficients held in registers and handed across segments
+    through LDS, divdamp's wind update applied by the momentum kernel) is
+    bit-identical to the one-lane sweep in the reference's order: level counts
+    that leave segments empty (3), uneven (9, 17), one short of full (60, the
+    benchmark's), exactly full (65 = 8 x 8 + 1) and past what the segments
+    hold (66, where the library falls back to the sweep)."""
+    c = small_config2(45, 7, kz)
+    hs = dm.build_case(c)
+    print(f"segmented vs one-lane kz={kz} prec={prec}", flush=True)
+    a = dm.run_case(hs, dm.case_params(c, precision=prec, unfused=1), 2)
+    b = dm.run_case(hs, dm.case_params(c, precision=prec, unfused=0), 2)
+    for n in PROGNOSTIC:
+        np.testing.assert_array_equal(dm.interior(a[n]), dm.interior(b[n]), err_msg=n)
+    if prec == dm.MLC_FP64 and kz == 60:
+        _check_fp64(b, run_oracle(hs, dm.case_params(c), 2), 3, "segmented kz=60 vs oracle", PROGNOSTIC)
+
+
 def test_case_load_matches_host_build():
     c = small_config2(33, 29, 9)
     hs = dm.build_case(c)
