@@ -1,0 +1,111 @@
+"""Properties of the step at the benchmark's full size, 2048 x 2048 x 60.
+
+The CPU oracle cannot run this grid in test time, so the full-size runs are
+held to properties that do not depend on size and that any indexing, tiling
+or halo mistake at this size would break:
+
+* the fused kernels (grouped advection, segmented column solve) give the
+  same bits as the one-kernel-per-phase path that follows the reference's
+  loop nests (and is itself held to the oracle on small grids);
+* the step commutes with a periodic shift of the whole state and geometry
+  (MPI halo order and no Coriolis force, the conditions under which the
+  reference's step is translation-equivariant): shifting the input by
+  (517, 1031) columns shifts the output by the same, bit for bit.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import dynamical_model_amd as dm
+from dynamical_model_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+PROGNOSTIC = ["u", "v", "w", "pai", "tetav", "qv"]
+SHIFT = (517, 1031)  # (j, i) columns; crosses every tile and strip boundary
+
+
+def _case(**over):
+    return dm.case_config(4, **over)  # 2048 x 2048 x 60, hills, jet, RH 60 %
+
+
+def _step_device_case(case, params, nsteps=1):
+    m = dm.Moloch()
+    m.allocate_moloch(case.jx, case.iy, case.kz, params)
+    m.load_case_device(case)
+    for n in range(1, nsteps + 1):
+        m.moloch(n, False)
+    m.sync()
+    return m
+
+
+def test_fullsize_fused_equals_unfused():
+    case = _case()
+    print(f"full size {case.jx}x{case.iy}x{case.kz} FP64: fused vs unfused, one step", flush=True)
+    a = _step_device_case(case, dm.case_params(case, precision=dm.MLC_FP64, unfused=0))
+    b = _step_device_case(case, dm.case_params(case, precision=dm.MLC_FP64, unfused=1))
+    try:
+        for n in PROGNOSTIC:
+            fa, fb = dm.interior(a.get_field(n)), dm.interior(b.get_field(n))
+            assert np.isfinite(fa).all(), n
+            np.testing.assert_array_equal(fa, fb, err_msg=n)
+    finally:
+        a.close()
+        b.close()
+
+
+def _wrap(core: np.ndarray) -> np.ndarray:
+    """Canonical array (two-point periodic halo) around interior points."""
+    return np.pad(core, [(0, 0)] * (core.ndim - 2) + [(2, 2), (2, 2)], mode="wrap")
+
+
+def _periodic(a: np.ndarray) -> np.ndarray:
+    return _wrap(a[..., 2:-2, 2:-2])
+
+
+def _shift(a: np.ndarray) -> np.ndarray:
+    dj, di = SHIFT
+    return _wrap(np.roll(a[..., 2:-2, 2:-2], (di, dj), axis=(-2, -1)))
+
+
+def test_fullsize_translation_equivariance():
+    case = _case(zero_coriolis=1)
+    params = dm.case_params(case, precision=dm.MLC_FP32, halo_mpi=1)
+    print(f"full size {case.jx}x{case.iy}x{case.kz} FP32: step(shift(x)) vs shift(step(x))", flush=True)
+    # the initial state as the device builder makes it, shifted on the host
+    m0 = dm.Moloch()
+    m0.allocate_moloch(case.jx, case.iy, case.kz, params)
+    m0.load_case_device(case)
+    small = dm.case_config(4, jx=3, iy=3, zero_coriolis=1)  # same levels (they depend on kz, ztop, zh only)
+    levels = dm.build_case(small).levels
+    a = dm.Moloch()
+    b = dm.Moloch()
+    try:
+        a.allocate_moloch(case.jx, case.iy, case.kz, params)
+        b.allocate_moloch(case.jx, case.iy, case.kz, params)
+        for name in _lib.FIELDS:
+            f = _periodic(m0.get_field(name))
+            a.set_field(name, f)
+            b.set_field(name, _shift(f))
+            del f
+        m0.close()
+        for name in _lib.LEVELS:
+            a.set_levels(name, levels[name])
+            b.set_levels(name, levels[name])
+        a.init_moloch()
+        b.init_moloch()
+        a.moloch(1, False)
+        b.moloch(1, False)
+        a.sync()
+        b.sync()
+        for n in PROGNOSTIC:
+            fa = a.get_field(n)
+            assert np.isfinite(dm.interior(fa)).all(), n
+            want = dm.interior(_shift(fa))
+            got = dm.interior(b.get_field(n))
+            np.testing.assert_array_equal(got, want, err_msg=n)
+    finally:
+        m0.close()
+        a.close()
+        b.close()
