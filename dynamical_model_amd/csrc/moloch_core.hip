@@ -1709,11 +1709,17 @@ typedef __attribute__((address_space(3))) void lds_void;
 // LDS byte address lds + 16 l (lds wave-uniform).  Issued from inline asm so
 // that the compiler's wait insertion, which cannot tell the buffers apart,
 // adds no wait for it; the kernels wait with wait_vm before reading the
-// buffer, and before they end.  No output registers.
+// buffer, and before they end.  The LDS address goes through m0, which the
+// compiler reserves: the asm saves m0 in a scalar temporary and restores it,
+// so the compiler's own m0 value survives.  No vector output registers.
 __device__ __forceinline__ void dma16(const void *gptr, const void *lds_ptr) {
   const unsigned lds = __builtin_amdgcn_readfirstlane((unsigned)(unsigned long)(const lds_void *)lds_ptr);
-  __asm__ volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(gptr), "s"(lds)
-                   : "memory", "m0");
+  unsigned save;
+  __asm__ volatile(
+      "s_mov_b32 %0, m0\n\ts_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off\n\ts_nop 0\n\ts_mov_b32 m0, %0"
+      : "=&s"(save)
+      : "v"(gptr), "s"(lds)
+      : "memory");
 }
 
 // wait until at most n vector-memory loads of this wave are outstanding
