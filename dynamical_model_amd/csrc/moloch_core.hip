@@ -417,19 +417,24 @@ __global__ void __launch_bounds__(256) k_sound_b(Dev<T> D, Par P, double dts, in
 // (segment 0, the top one, also takes level 1); a block holds CW columns of
 // one row and one lane per (column, segment), 64 / CW segments per wave.
 // Each lane keeps what the column solve and the theta/pi update need for its
-// levels in registers: the explicit right-hand side and the two coupling
-// coefficients (FP64), the divergence, pi, and with KEEP also theta and the
-// old w.  The forward elimination (bottom segment first) and the back
+// levels on chip: in registers the explicit right-hand side and the two
+// coupling coefficients (FP64), the divergence and pi; in LDS theta, the old
+// w, qv and fmz.  The forward elimination (bottom segment first) and the back
 // substitution (top segment first) pass through the segments in turn,
 // handing the (g, c) pair and then w across segment boundaries through LDS,
 // one block barrier per segment: the operations and their order are those of
 // the one-lane sweep of k_sound_b, so the results are bit-identical, while
 // neither the elimination coefficients nor the divergence go through HBM.
 // Needs 2 <= kz <= NS*L + 1; nbj = number of CW-column tiles per row.
-template <typename T, int NS, int L, int CW, bool KEEP>
+template <typename T, int NS, int L, int CW>
 __global__ void __launch_bounds__(NS * CW) k_sound_bs(Dev<T> D, Par P, double dts, int ns, int split, int nbj) {
   static_assert(64 % CW == 0 && (NS * CW) % 64 == 0, "whole waves of whole columns");
   __shared__ double hg[NS][CW], hc[NS][CW], hw[NS][CW], hw1[NS][CW];
+  // theta, old w, qv and fmz of each lane's levels, kept from the first
+  // phase to the last (in registers they would cost occupancy, read again
+  // from memory they would cost traffic)
+  __shared__ T stash[4][L][NS * CW];
+  const int tl = threadIdx.y * 64 + threadIdx.x;
   const Geo &g = D.g;
   const int lane = threadIdx.x % CW;                           // column within the block
   const int wv = threadIdx.y * (64 / CW) + threadIdx.x / CW;  // segment
@@ -465,7 +470,7 @@ __global__ void __launch_bounds__(NS * CW) k_sound_bs(Dev<T> D, Par P, double dt
   };
 
   double rz[L], ru[L], rd[L];  // zwexpl, zu, zd; then g, c; then the new w
-  T zz[L], wo[L], th[L];
+  T zz[L];
   double pp[L];
   // level a-1 (the segment's upper neighbour; level 1 for wave 0)
   T zt = 0, tht = 0, fzt = 0;
@@ -486,19 +491,19 @@ __global__ void __launch_bounds__(NS * CW) k_sound_bs(Dev<T> D, Par P, double dt
       const long xk = g.at(j, i, k), xm = xk - g.plane;
       const T zkt = level_z(xk);
       zz[l] = zkt;
-      const T tkt = D.tetav[xk], wkt = D.w[xk];
-      if (KEEP) {
-        th[l] = tkt;
-        wo[l] = wkt;
-      }
+      const T tkt = D.tetav[xk], wkt = D.w[xk], qvt = D.qv[xk], fzt_k = D.fmz[xk];
+      stash[0][l][tl] = tkt;
+      stash[1][l][tl] = wkt;
+      stash[2][l][tl] = qvt;
+      stash[3][l][tl] = fzt_k;
       pp[l] = D.pai[xk];
-      const double zk = (double)zkt, tk = (double)tkt, pk = pp[l], fzk = (double)D.fmz[xk];
+      const double zk = (double)zkt, tk = (double)tkt, pk = pp[l], fzk = (double)fzt_k;
       const double qsk = (double)D.qsat[xk];
       const double wk = (double)wkt;
       const double ffk = (double)D.fmzf[xk];
       double zrom1w = 0.5 * kCpd * ffk * (tkm + tk);
       zrom1w = zrom1w - kCpd * wk * ffk * ffk * (double)ns * dtrdz * (tkm - tk);
-      if ((double)D.qv[xk] > 0.96 * qsk && wk > 0.1) {
+      if ((double)qvt > 0.96 * qsk && wk > 0.1) {
         double zqs = 0.5 * (qsk + qskm);
         double tm = (double)D.t[xm];
         double zdth = kEgrav * wk * (double)(ns - 1) * dts * kWlhv * kWlhv * zqs / (kCpd * pkm * kRwat * tm * tm);
@@ -567,9 +572,9 @@ __global__ void __launch_bounds__(NS * CW) k_sound_bs(Dev<T> D, Par P, double dt
   if (cnt == 0) return;
 
   // w, deltaw; theta and pi of each level once w above and below are known
-  auto finish = [&](long x, int k, T zk, T fzk, T thk, double po, T wtop, T wbot) {
+  auto finish = [&](long x, int k, T zk, T fzk, T thk, T qv, double po, T wtop, T wbot) {
     T zf = zk + dtrdz_t * fzk * (wtop - wbot);
-    T qv = D.qv[x], qc = D.qc[x];
+    T qc = D.qc[x];
     if (P.fulleq) {
       zf = zf * (T(1) + T(0.86) * qv + T(3.2) * qc) / (T(1) + T(0.96) * qv + T(4.8) * qc);
       T tet = thk * (T(1) + (T)kRdrcv * zf * (T(0.25) * qv + T(4.2) * D.qwltot[x] + T(2.1) * D.qwitot[x]));
@@ -587,19 +592,19 @@ __global__ void __launch_bounds__(NS * CW) k_sound_bs(Dev<T> D, Par P, double dt
   };
   if (!act) return;
   const T wnext = bl == nk ? D.w[surf] : (T)hw1[wv + 1 < NS ? wv + 1 : wv][lane];
-  if (wv == 0) finish(g.at(j, i, 1), 1, zt, fzt, tht, ppt, w1t, (T)rz[0]);
+  if (wv == 0) finish(g.at(j, i, 1), 1, zt, fzt, tht, D.qv[g.at(j, i, 1)], ppt, w1t, (T)rz[0]);
 #pragma unroll
   for (int l = 0; l < L; l++) {
     if (l < cnt) {
       const int k = a + l;
       const long x = g.at(j, i, k);
       const T wn_t = (T)rz[l];
-      T dw = -(KEEP ? wo[l] : D.w[x]) + wn_t;
+      T dw = -stash[1][l][tl] + wn_t;
       D.w[x] = wn_t;
       D.deltaw[x] = dw;
       store_faces(D.deltaw, g, j, i, k, dw);
       const T wb = l + 1 < cnt ? (T)rz[l + 1 < L ? l + 1 : l] : wnext;
-      finish(x, k, zz[l], D.fmz[x], KEEP ? th[l] : D.tetav[x], pp[l], wn_t, wb);
+      finish(x, k, zz[l], stash[3][l][tl], stash[0][l][tl], stash[2][l][tl], pp[l], wn_t, wb);
     }
   }
 }
@@ -2696,11 +2701,8 @@ struct Core : CoreBase {
     int split = prm.divfilter ? 1 : 0;
     const bool seg = segmented_solve();
     if (seg) {
-      // FP32: theta and the old w are read again at the end rather than held
-      // (166 VGPRs, three waves per SIMD); FP64 holds them (two either way).
-      constexpr bool keep = sizeof(T) == sizeof(double);
       const int nbj = (g.nj + SEG_CW - 1) / SEG_CW;
-      launch(KK_SOUND_B, fbytes(13, 2), k_sound_bs<T, SEG_NS, SEG_L, SEG_CW, keep>, dim3(nbj * g.ni),
+      launch(KK_SOUND_B, fbytes(13, 2), k_sound_bs<T, SEG_NS, SEG_L, SEG_CW>, dim3(nbj * g.ni),
              dim3(64, SEG_NS * SEG_CW / 64), D, P, dts, ns, split, nbj);
     } else {
       launch(KK_SOUND_B, fbytes(13, 2), k_sound_b<T>, col_grid(), blk(), D, P, dts, ns, split);
