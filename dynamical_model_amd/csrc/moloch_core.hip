@@ -103,6 +103,10 @@ struct Dev {
   double *pf, *smd;       // filter scratch
   double *thg, *thc;      // tridiagonal elimination (FP64)
   double *diag;           // reduction partials
+  // increments of the substep being run (its result minus its start) of u,
+  // v, w (level k holds w(k+1)) and pi, written by the sound kernels when
+  // not null: the learned corrector's inputs without phase-start copies
+  float *iu, *iv, *iw, *ipai;
 };
 
 // ------------------------------------------------------------ device math
@@ -259,6 +263,7 @@ __global__ void __launch_bounds__(256) k_sound_a(Dev<T> D, Par P, double dts) {
       T zuh = u0 * hx0 + u1 * hx1;
       T zvh = v0 * hy0 + v1 * hy1;
       T wb = T(0.5) * (zuh + zvh);
+      if (D.iw) D.iw[g.at(j, i, nk)] = (float)(wb - D.w[g.at(j, i, nk + 1)]);
       D.w[g.at(j, i, nk + 1)] = wb;
       D.s[g.at(j, i, nk + 1)] = -wb;
     }
@@ -589,6 +594,7 @@ __global__ void __launch_bounds__(NS * CW) k_sound_bs(Dev<T> D, Par P, double dt
     double pn = po * (1.0 - kRdrcv * (double)zf);
     D.pai[x] = pn;
     if (P.halo_mpi) store_faces(D.pai, g, j, i, k, pn);
+    if (D.ipai) D.ipai[x] = (float)(pn - po);
   };
   if (!act) return;
   const T wnext = bl == nk ? D.w[surf] : (T)hw1[wv + 1 < NS ? wv + 1 : wv][lane];
@@ -602,6 +608,7 @@ __global__ void __launch_bounds__(NS * CW) k_sound_bs(Dev<T> D, Par P, double dt
       T dw = -stash[1][l][tl] + wn_t;
       D.w[x] = wn_t;
       D.deltaw[x] = dw;
+      if (D.iw) D.iw[x - g.plane] = (float)dw;
       store_faces(D.deltaw, g, j, i, k, dw);
       const T wb = l + 1 < cnt ? (T)rz[l + 1 < L ? l + 1 : l] : wnext;
       finish(x, k, zz[l], stash[3][l][tl], stash[0][l][tl], stash[2][l][tl], pp[l], wn_t, wb);
@@ -669,6 +676,7 @@ __global__ void __launch_bounds__(256) k_sound_c(Dev<T> D, Par P, double dts) {
     double gzh = D.gzitakh[k - 1];
     T u_c = D.u[x], u_im = D.u[x - pj], u_jp = D.u[x + 1], u_jpim = D.u[x + 1 - pj];
     T v_c = D.v[x], v_jm = D.v[x - 1], v_jmip = D.v[x - 1 + pj], v_ip = D.v[x + pj];
+    const T u_start = u_c, v_start = v_c;
     if (DAMP) {
       const T xk = (T)D.xknu[k - 1];
       const T z00 = D.zdh[x], zm0 = D.zdh[x - 1], zp0 = D.zdh[x + 1];
@@ -707,6 +715,7 @@ __global__ void __launch_bounds__(256) k_sound_c(Dev<T> D, Par P, double dts) {
       double zcor1u = coru * dts * 0.25 * ((double)v_c + (double)v_jm + (double)v_jmip + (double)v_ip);
       double un = (double)u_c + zcor1u - zfz * hx * gzh - zcx * zrom1u * (pc - D.pai[x - 1]);
       D.un[x] = (T)un;
+      if (DAMP && D.iu) D.iu[x] = (float)((T)un - u_start);
     }
     // Equation 18
     {
@@ -715,6 +724,7 @@ __global__ void __launch_bounds__(256) k_sound_c(Dev<T> D, Par P, double dts) {
       double zcor1v = corv * dts * 0.25 * ((double)u_c + (double)u_im + (double)u_jp + (double)u_jpim);
       double vn = (double)v_c - zcor1v - zfz * hy * gzh - zcy * zrom1v * (pc - D.pai[x - pj]);
       D.vn[x] = (T)vn;
+      if (DAMP && D.iv) D.iv[x] = (float)((T)vn - v_start);
     }
     dwc = dwc1;
     dwjm = dwjm1;
@@ -1436,11 +1446,15 @@ struct LacRows {
   }
 };
 
-// Phase-start copies of the fields whose increments are inputs.
+// The increments that are network inputs: from phase-start copies of u, v,
+// w, pi (inc = 0), or as the sound kernels wrote them (inc = 1: iu, iv, iw,
+// ipai, the same float values).
 template <typename T>
 struct LacSnap {
   const T *u, *v, *w;
   const double *pai;
+  const float *iu, *iv, *iw, *ipai;
+  int inc;
 };
 
 template <typename T>
@@ -1451,6 +1465,10 @@ __device__ __forceinline__ float lac_value(const Dev<T> &D, const LacSnap<T> &S,
   j = j < 1 ? j + g.nj : (j > g.nj ? j - g.nj : j);
   i = i < 1 ? i + g.ni : (i > g.ni ? i - g.ni : i);
   const long x = g.at(j, i, k);
+  if (S.inc && f < 4) {
+    const float *d = f == 0 ? S.iu : f == 1 ? S.iv : f == 2 ? S.iw : S.ipai;
+    return d[x];
+  }
   switch (f) {
     case 0: return (float)(D.u[x] - S.u[x]);
     case 1: return (float)(D.v[x] - S.v[x]);
@@ -2307,7 +2325,7 @@ struct Core : CoreBase {
   mlc_params prm;
   Geo g;
   hipStream_t stream = nullptr;
-  Dev<T> D;
+  Dev<T> D{};
   std::vector<void *> allocs;
   std::vector<double> zita, zitah, ffilt;                   // host levels
   std::vector<double> h_msfu, h_msfv, h_msfx, h_ulat, h_vlat;  // host geometry
@@ -2943,10 +2961,25 @@ struct Core : CoreBase {
     copy_field(pai, D.pai, g.nk);
   }
 
-  LacSnap<T> lac_snap() const { return LacSnap<T>{L.su, L.sv, L.sw, L.spai}; }
+  LacSnap<T> lac_snap(bool inc = false) const {
+    return LacSnap<T>{L.su, L.sv, L.sw, L.spai, (const float *)L.su, (const float *)L.sv, (const float *)L.sw,
+                      (const float *)L.spai, inc ? 1 : 0};
+  }
+  // Increment mode: the long substep's kernels write the increments of u, v,
+  // w, pi into the snapshot buffers (as floats), so a learned phase without a
+  // check needs no phase-start copies.  Only on the segmented path (the
+  // one-lane sweep updates the winds in place) and without the divergence
+  // filter (which updates pi in a kernel of its own).
+  bool lac_increments_ok() const { return segmented_solve() && !prm.divfilter && L.cfg.engine == 1; }
+  void lac_increments(bool on) {
+    D.iu = on ? (float *)L.su : nullptr;
+    D.iv = on ? (float *)L.sv : nullptr;
+    D.iw = on ? (float *)L.sw : nullptr;
+    D.ipai = on ? (float *)L.spai : nullptr;
+  }
 
   // Network outputs for every row from the current state and the snapshot.
-  int lac_infer() {
+  int lac_infer(bool inc) {
     const float *st = L.stat;
     const double nh = L.cfg.nhid;
     int rc = mlp_reserve_rows(L.net, L.chunk);
@@ -2955,7 +2988,7 @@ struct Core : CoreBase {
     if (L.cfg.engine == 1) {
       const float *fs = L.stat + 2 * L.nin + 2 * L.nout;
       launch(KK_LAC_GATHER, fbytes(6, 2) + (double)(L.nflp / 16) * L.fp.plane() * sizeof(__bf16),
-             k_lac_features<T>, dim3((L.fp.fpj + 63) / 64, (L.fp.fpi + 3) / 4), dim3(256), D, lac_snap(), L.fp,
+             k_lac_features<T>, dim3((L.fp.fpj + 63) / 64, (L.fp.fpi + 3) / 4), dim3(256), D, lac_snap(inc), L.fp,
              L.nflp / 16, fs, fs + L.nflp);
     }
     for (long r0 = 0; r0 < L.rtot; r0 += L.chunk) {
@@ -2966,7 +2999,7 @@ struct Core : CoreBase {
         continue;
       }
       timed(KK_LAC_GATHER, (double)n * L.nin * sizeof(__bf16) + fbytes(6, 0) * n / L.rtot, [&]() {
-        hipLaunchKernelGGL(k_lac_gather<T>, dim3(n / 64), dim3(256), 0, stream, D, lac_snap(), L.rows,
+        hipLaunchKernelGGL(k_lac_gather<T>, dim3(n / 64), dim3(256), 0, stream, D, lac_snap(inc), L.rows,
                            (const int *)nullptr, r0, n, L.nin, (float *)nullptr, L.Xc, st, st + L.nin);
       });
       timed(KK_LAC_MLP, flops, [&]() { rc = mlp_forward_dev(L.net, L.Xc, L.nin, n, L.Y + r0 * L.nout, L.nout); });
@@ -3041,9 +3074,14 @@ struct Core : CoreBase {
     if (!prm.fulleq) halo(D.tetav, g.nk, 0);
     const int nb = prm.nsound - 1;
     int rc;
-    lac_snapshot(check);
+    const bool inc = correct && !check && lac_increments_ok();
+    if (inc)
+      lac_increments(true);
+    else
+      lac_snapshot(check);
     substep(dts * nb, 1);
-    if (correct && (rc = lac_infer())) return rc;
+    lac_increments(false);
+    if (correct && (rc = lac_infer(inc))) return rc;
     if (check) lac_save(L.ru, L.rv, L.rw, L.rpai);
     if (correct) lac_apply();
     if (check) {
