@@ -1011,7 +1011,8 @@ __device__ __forceinline__ Q waf_vflux(Q s, Q dtrdz, Q qm2, Q qm1, Q q0, Q q1) {
 
 // ============================================= multi-field wafone, 2 kernels
 // The advected set (:911-920) goes through wafone in groups of up to MAXF
-// fields of one storage type.  k_wafv runs both vertical half passes for
+// fields of one storage type; in the FP32 core the FP64 Exner function rides
+// along with the last group in the horizontal kernel (FieldD).  k_wafv runs both vertical half passes for
 // every field of the group, one column per lane, sharing the loads of s,
 // fmz, fmzf and the zrfmu/zrfmd divisions between the fields.  k_wafh then
 // does the meridional and zonal sweeps of every field of the group for one
@@ -1026,51 +1027,27 @@ struct FieldSet {
   Q *out[MAXF];       // after the horizontal sweeps
 };
 
-template <typename T, typename Q, int NF>
-__global__ void __launch_bounds__(256) k_wafv(Dev<T> D, Par P, FieldSet<Q> F, double dta) {
-  COL_IDX
-  const int nk = g.nk;
-  const Q dtrdz = Q(0.5) * (Q)(dta / P.dzita);
-  const long c0 = g.at(j, i, 1);
-  const long pl = g.plane;
-  // windows: S interfaces m-4..m, ZU/ZD levels m-4..m (slot 4 = m);
-  // Pw levels m-3..m (slot 3 = m); Ww first-pass levels m-5..m-2
-  Q S[5], ZU[5], ZD[5], ffz;
-  Q Pw[NF][4], Ww[NF][4], F1[NF], F2[NF];
-  // the loads of iteration m (s at interface m, fmz and the fields at level
-  // min(m, kz), fmzf below it), issued one iteration ahead; past the column
-  // they repeat the last level and are not used
-  Q n_s, n_fz, n_ff, n_p[NF];
-  auto fetch = [&](int m) {
-    const int mm = m <= nk ? m : nk, ms = m <= nk + 1 ? m : nk + 1;
-    const long xm = c0 + (long)(mm - 1) * pl;
-    n_s = (Q)D.s[c0 + (long)(ms - 1) * pl];
-    n_fz = (Q)D.fmz[xm];
-    n_ff = (Q)D.fmzf[xm + pl];
+// An FP64 field riding along with a group of FP32 fields (the Exner function
+// of the FP32 core): advected with every coefficient evaluated in FP64 from
+// the same loads, i.e. exactly as in a group of its own.
+struct FieldD {
+  const double *in;
+  double *wz;
+  double *out;
+};
+
+// Field-independent window of the vertical passes in precision R: s at
+// interfaces m-4..m, zrfmu / zrfmd (:1010-1011) of levels m-4..m.
+template <typename R>
+struct VCoef {
+  R S[5], ZU[5], ZD[5], ffz, dtrdz;
+  __device__ __forceinline__ void init(R ffz0, double dta, double dzita) {
+    dtrdz = R(0.5) * (R)(dta / dzita);
+    ffz = ffz0;
 #pragma unroll
-    for (int f = 0; f < NF; f++) n_p[f] = F.in[f][xm];
-  };
-  fetch(1);
-  for (int m = 1; m <= nk + 4; m++) {
-    const Q sm = m <= nk + 1 ? n_s : Q(0);
-    const Q fzm = n_fz, ff1 = n_ff;
-    Q pin[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) pin[f] = n_p[f];
-    fetch(m + 1);
-    if (m == 1) {
-      ffz = (Q)D.fmzf[c0];
-#pragma unroll
-      for (int q = 0; q < 5; q++) S[q] = Q(0), ZU[q] = Q(0), ZD[q] = Q(0);
-#pragma unroll
-      for (int f = 0; f < NF; f++) {
-        Q p1 = pin[f];
-#pragma unroll
-        for (int q = 0; q < 4; q++) Pw[f][q] = p1;
-        F1[f] = Q(0);
-        F2[f] = Q(0);
-      }
-    }
+    for (int q = 0; q < 5; q++) S[q] = R(0), ZU[q] = R(0), ZD[q] = R(0);
+  }
+  __device__ __forceinline__ void advance(int m, int nk, R sm, R fzm, R ff1) {
 #pragma unroll
     for (int q = 0; q < 4; q++) S[q] = S[q + 1], ZU[q] = ZU[q + 1], ZD[q] = ZD[q + 1];
     S[4] = sm;
@@ -1079,44 +1056,102 @@ __global__ void __launch_bounds__(256) k_wafv(Dev<T> D, Par P, FieldSet<Q> F, do
       ZD[4] = dtrdz * fzm / ff1;  // zrfmd (:1011)
       ffz = ff1;
     }
+  }
+};
+
+// One field's window of the two vertical half passes: levels m-3..m of the
+// field (Pw), first-pass levels m-5..m-2 (Ww), the last fluxes.  step(m)
+// takes the field at level min(m, kz) and stores the result of level m-4.
+template <typename R>
+struct VField {
+  R Pw[4], Ww[4], F1, F2;
+  __device__ __forceinline__ void init(R p1) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) Pw[q] = p1;
+    F1 = R(0);
+    F2 = R(0);
+  }
+  __device__ __forceinline__ void step(const VCoef<R> &c, int m, int nk, R pm, R *wz, long c0, long pl) {
     const int kb = m - 2, kc = m - 4;
 #pragma unroll
-    for (int f = 0; f < NF; f++) {
-      Q pm = pin[f];
-#pragma unroll
-      for (int q = 0; q < 3; q++) Pw[f][q] = Pw[f][q + 1];
-      Pw[f][3] = pm;
-      // first pass, level kb from interfaces kb (F1) and kb+1 = m-1
-      if (kb >= 1 && kb <= nk) {
-        Q f1 = Q(0);
-        if (m - 1 >= 2 && m - 1 <= nk) f1 = waf_vflux(S[3], dtrdz, Pw[f][0], Pw[f][1], Pw[f][2], Pw[f][3]);
-        Q q = Pw[f][1];
-        Q zdv = (S[2] * ZU[2] - S[3] * ZD[2]) * q;
-        Q w1 = q - F1[f] * ZU[2] + f1 * ZD[2] + zdv;
-        F1[f] = f1;
-        if (kb == 1) {
-          Ww[f][0] = Ww[f][1] = Ww[f][2] = Ww[f][3] = w1;
-        } else {
-          Ww[f][0] = Ww[f][1];
-          Ww[f][1] = Ww[f][2];
-          Ww[f][2] = Ww[f][3];
-          Ww[f][3] = w1;
-        }
-      } else if (kb > nk) {
-        Ww[f][0] = Ww[f][1];
-        Ww[f][1] = Ww[f][2];
-        Ww[f][2] = Ww[f][3];
+    for (int q = 0; q < 3; q++) Pw[q] = Pw[q + 1];
+    Pw[3] = pm;
+    // first pass, level kb from interfaces kb (F1) and kb+1 = m-1
+    if (kb >= 1 && kb <= nk) {
+      R f1 = R(0);
+      if (m - 1 >= 2 && m - 1 <= nk) f1 = waf_vflux(c.S[3], c.dtrdz, Pw[0], Pw[1], Pw[2], Pw[3]);
+      R q = Pw[1];
+      R zdv = (c.S[2] * c.ZU[2] - c.S[3] * c.ZD[2]) * q;
+      R w1 = q - F1 * c.ZU[2] + f1 * c.ZD[2] + zdv;
+      F1 = f1;
+      if (kb == 1) {
+        Ww[0] = Ww[1] = Ww[2] = Ww[3] = w1;
+      } else {
+        Ww[0] = Ww[1];
+        Ww[1] = Ww[2];
+        Ww[2] = Ww[3];
+        Ww[3] = w1;
       }
-      // second pass, level kc from interfaces kc (F2) and kc+1 = m-3
-      if (kc >= 1 && kc <= nk) {
-        Q f2 = Q(0);
-        if (m - 3 >= 2 && m - 3 <= nk) f2 = waf_vflux(S[1], dtrdz, Ww[f][0], Ww[f][1], Ww[f][2], Ww[f][3]);
-        Q q = Ww[f][1];
-        Q zdv = (S[0] * ZU[0] - S[1] * ZD[0]) * q;
-        F.wz[f][c0 + (long)(kc - 1) * pl] = q - F2[f] * ZU[0] + f2 * ZD[0] + zdv;
-        F2[f] = f2;
-      }
+    } else if (kb > nk) {
+      Ww[0] = Ww[1];
+      Ww[1] = Ww[2];
+      Ww[2] = Ww[3];
     }
+    // second pass, level kc from interfaces kc (F2) and kc+1 = m-3
+    if (kc >= 1 && kc <= nk) {
+      R f2 = R(0);
+      if (m - 3 >= 2 && m - 3 <= nk) f2 = waf_vflux(c.S[1], c.dtrdz, Ww[0], Ww[1], Ww[2], Ww[3]);
+      R q = Ww[1];
+      R zdv = (c.S[0] * c.ZU[0] - c.S[1] * c.ZD[0]) * q;
+      wz[c0 + (long)(kc - 1) * pl] = q - F2 * c.ZU[0] + f2 * c.ZD[0] + zdv;
+      F2 = f2;
+    }
+  }
+};
+
+// Both vertical half passes (:951-1073) of NF fields of type Q, one column
+// per lane, k swept as a sliding window; the loads of s, fmz, fmzf and the
+// zrfmu/zrfmd divisions are shared by the fields.  (An FP64 rider of the
+// horizontal kernel gets a launch of its own here: in one kernel its FP64
+// window and coefficients cost half the occupancy.)
+template <typename T, typename Q, int NF>
+__global__ void __launch_bounds__(256) k_wafv(Dev<T> D, Par P, FieldSet<Q> F, double dta) {
+  COL_IDX
+  const int nk = g.nk;
+  const long c0 = g.at(j, i, 1);
+  const long pl = g.plane;
+  VCoef<Q> cq;
+  VField<Q> vf[NF];
+  // the loads of iteration m (s at interface m, fmz and the fields at level
+  // min(m, kz), fmzf below it), issued one iteration ahead; past the column
+  // they repeat the last level and are not used
+  T n_s, n_fz, n_ff;
+  Q n_p[NF];
+  auto fetch = [&](int m) {
+    const int mm = m <= nk ? m : nk, ms = m <= nk + 1 ? m : nk + 1;
+    const long xm = c0 + (long)(mm - 1) * pl;
+    n_s = D.s[c0 + (long)(ms - 1) * pl];
+    n_fz = D.fmz[xm];
+    n_ff = D.fmzf[xm + pl];
+#pragma unroll
+    for (int f = 0; f < NF; f++) n_p[f] = F.in[f][xm];
+  };
+  fetch(1);
+  cq.init((Q)D.fmzf[c0], dta, P.dzita);
+  for (int m = 1; m <= nk + 4; m++) {
+    const T sm = m <= nk + 1 ? n_s : T(0);
+    const T fzm = n_fz, ff1 = n_ff;
+    Q pin[NF];
+#pragma unroll
+    for (int f = 0; f < NF; f++) pin[f] = n_p[f];
+    fetch(m + 1);
+    if (m == 1) {
+#pragma unroll
+      for (int f = 0; f < NF; f++) vf[f].init(pin[f]);
+    }
+    cq.advance(m, nk, (Q)sm, (Q)fzm, (Q)ff1);
+#pragma unroll
+    for (int f = 0; f < NF; f++) vf[f].step(cq, m, nk, pin[f], F.wz[f], c0, pl);
   }
 }
 
@@ -1146,8 +1181,141 @@ __device__ __forceinline__ int wrap_idx(int x, int n) { return ((x - 1) % n + n)
 // r+1 are issued before row r is evaluated.  Everything that does not depend
 // on the advected field (Courant numbers, fmz ratios, the divergence-
 // correction coefficients) is evaluated once per row for all fields.
-template <typename T, typename Q, int NF, bool ROT>
-__global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, double dta, int ntj, int nti) {
+// meridional flux through the south face of a row (Courant number mz,
+// amplitude ma) from wz of rows c-2, c-1, c, c+1 (:1082-1131)
+template <typename R>
+__device__ __forceinline__ R waf_mflux(R mz, R ma, R wmm, R wm, R wc, R wp) {
+  R is, a, b;
+  if (mz > R(0)) {
+    is = R(1);
+    a = wm;
+    b = wmm;
+  } else {
+    is = R(-1);
+    a = wp;
+    b = wc;
+  }
+  const R zzden = sgn_floor(wc - wm);
+  const R rq = (a - b) / zzden;
+  const R bb = waf_b(rq);
+  const R zphi = is + mz * bb - is * bb;
+  return ma * ((R(1) + zphi) * wm + (R(1) - zphi) * wc);
+}
+
+// field-independent loads of one row of a strip (storage type T), the
+// coefficients made of them in precision R
+template <typename T>
+struct HRow {
+  T vn, fn, fe, fw, uc, up, rvn, m0, m1, m2;
+};
+template <typename R>
+struct HCoef {
+  R mzn, man, pfs, pfn, pcd, pm2, zz, za, zfw, zfe, zcd, zm2;
+};
+template <typename R, bool ROT, typename T>
+__device__ __forceinline__ HCoef<R> waf_hcoef(const HRow<T> &ci, T v_c, T rv_c, T f_s, T f_c, R dtrdx, R dtrdy) {
+  HCoef<R> c;
+  const R vn = (R)ci.vn, rvn = (R)ci.rvn;
+  if constexpr (ROT) {
+    c.mzn = vn * dtrdy;
+    c.man = R(0.5) * vn;
+  } else {
+    c.mzn = vn * rvn * dtrdy;
+    c.man = R(0.5) * vn * rvn;
+  }
+  {
+    const R f0 = (R)f_c, fn = (R)ci.fn, fs = (R)f_s;
+    const R vc = (R)v_c, vp = vn;
+    if constexpr (ROT) {
+      const R mx = (R)ci.m0;
+      const R zhxvtn = dtrdy * rvn * mx;
+      const R zhxvts = dtrdy * (R)rv_c * mx;
+      c.pfn = zhxvtn * R(2) * f0 / (f0 + fn);
+      c.pfs = zhxvts * R(2) * f0 / (f0 + fs);
+      c.pcd = vp * c.pfn - vc * c.pfs;
+      c.pm2 = R(1);
+    } else {
+      c.pfn = dtrdy * R(2) * f0 / (f0 + fn);
+      c.pfs = dtrdy * R(2) * f0 / (f0 + fs);
+      c.pcd = vp * rvn * c.pfn - vc * (R)rv_c * c.pfs;
+      c.pm2 = (R)ci.m0;
+    }
+  }
+  {
+    const R uc = (R)ci.uc, up = (R)ci.up;
+    const R f0 = (R)f_c, fe = (R)ci.fe, fw = (R)ci.fw;
+    if constexpr (ROT) {
+      const R mu = (R)ci.m1;
+      c.zz = uc * mu * dtrdx;
+      c.za = R(0.5) * uc;
+      const R zcostx = dtrdx * mu;
+      c.zfe = zcostx * R(2) * f0 / (f0 + fe);
+      c.zfw = zcostx * R(2) * f0 / (f0 + fw);
+      c.zcd = up * c.zfe - uc * c.zfw;
+      c.zm2 = R(1);
+    } else {
+      const R rm = (R)ci.m1;
+      c.zz = uc * rm * dtrdx;
+      c.za = R(0.5) * uc * rm;
+      c.zfe = dtrdx * R(2) * f0 / (f0 + fe);
+      c.zfw = dtrdx * R(2) * f0 / (f0 + fw);
+      c.zcd = up * (R)ci.m2 * c.zfe - uc * rm * c.zfw;
+      c.zm2 = (R)ci.m0;
+    }
+  }
+  return c;
+}
+
+// One field, one row: the meridional flux at the north face and p0 of the
+// row (:1082-1131), then the zonal flux at each lane's west face and the
+// result (:1138-1187); the window of wz rows slides north.
+template <typename R, bool ROT>
+__device__ __forceinline__ R waf_hrow(R (&w)[5], R &zs, R pp, const HCoef<R> &c, int lm1, int lm2, int lp1) {
+  const R zn = waf_mflux(c.mzn, c.man, w[1], w[2], w[3], w[4]);
+  R p0;
+  const R zdv = c.pcd * pp;
+  if constexpr (ROT)
+    p0 = w[2] + zs * c.pfs - zn * c.pfn + zdv;
+  else
+    p0 = w[2] + c.pm2 * (zs * c.pfs - zn * c.pfn + zdv);
+  const R pm = __shfl(p0, lm1, 64), pmm = __shfl(p0, lm2, 64), ppl = __shfl(p0, lp1, 64);
+  R is, a, b;
+  if (c.zz > R(0)) {
+    is = R(1);
+    a = pm;
+    b = pmm;
+  } else {
+    is = R(-1);
+    a = ppl;
+    b = p0;
+  }
+  const R zzden = sgn_floor(p0 - pm);
+  const R rq = (a - b) / zzden;
+  const R bb = waf_b(rq);
+  const R zphi = is + c.zz * bb - is * bb;
+  const R zw = c.za * ((R(1) + zphi) * pm + (R(1) - zphi) * p0);
+  const R ze = __shfl(zw, lp1, 64);
+  const R zdu = c.zcd * pp;
+  R o;
+  if constexpr (ROT)
+    o = p0 + zw * c.zfw - ze * c.zfe + zdu;
+  else
+    o = p0 + c.zm2 * (zw * c.zfw - ze * c.zfe + zdu);
+  w[0] = w[1];
+  w[1] = w[2];
+  w[2] = w[3];
+  w[3] = w[4];
+  zs = zn;
+  return o;
+}
+
+// Both horizontal sweeps of NF fields of type Q and ND (0 or 1) FP64 fields
+// for one strip of one level per wave; the field-independent coefficients
+// are evaluated once per row and precision.
+template <typename T, typename Q, int NF, int ND, bool ROT>
+__global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, FieldD FD, double dta, int ntj,
+                                              int nti) {
+  static_assert(ND == 0 || ND == 1, "at most one FP64 rider");
   const Geo &g = D.g;
   const int nj = g.nj, ni = g.ni;
   const long total = (long)ntj * nti * g.nk;  // strips
@@ -1165,95 +1333,88 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, do
   const int jw = wrap_idx(ja, nj);
   const int jp = jw == nj ? 1 : jw + 1, jm = jw == 1 ? nj : jw - 1;
   const Q dtrdx = (Q)(dta / P.dx), dtrdy = (Q)(dta / P.dx);
+  const double dtrdx_d = dta / P.dx, dtrdy_d = dta / P.dx;
   const long pl = (long)(k - 1) * g.plane;
   auto at3 = [&](int jj, int ii) { return pl + (long)(ii + 1) * g.pj + (long)(jj + JOFF); };
   auto nxt = [&](int ii) { return ii == ni ? 1 : ii + 1; };
   const int lm1 = lane > 0 ? lane - 1 : 0, lm2 = lane > 1 ? lane - 2 : 0, lp1 = lane < 63 ? lane + 1 : 63;
 
-  // meridional flux through the south face of row c (Courant number mz,
-  // amplitude ma) from wz of rows c-2, c-1, c, c+1
-  auto mflux = [&](Q mz, Q ma, Q wmm, Q wm, Q wc, Q wp) -> Q {
-    Q is, a, b;
-    if (mz > Q(0)) {
-      is = Q(1);
-      a = wm;
-      b = wmm;
-    } else {
-      is = Q(-1);
-      a = wp;
-      b = wc;
-    }
-    const Q zzden = sgn_floor(wc - wm);
-    const Q rq = (a - b) / zzden;
-    const Q bb = waf_b(rq);
-    const Q zphi = is + mz * bb - is * bb;
-    return ma * ((Q(1) + zphi) * wm + (Q(1) - zphi) * wc);
-  };
-  // face coefficients of the south face of row ii
-  auto mface = [&](int ii, Q &mz, Q &ma) {
-    const Q vv = (Q)D.v[at3(jw, ii)];
-    if constexpr (ROT) {
-      mz = vv * dtrdy;
-      ma = Q(0.5) * vv;
-    } else {
-      const Q rm = (Q)D.rmv[g.at2(jw, ii)];
-      mz = vv * rm * dtrdy;
-      ma = Q(0.5) * vv * rm;
-    }
-  };
-
   // rows r-2 .. r+2 of the first row r = r0
   int i_m2 = wrap_idx(r0 - 2, ni);
   int i_m1 = nxt(i_m2), i_c = nxt(i_m1), i_p1 = nxt(i_c), i_p2 = nxt(i_p1);
   Q w[NF][5], zs[NF];
+  double wd[5], zsd = 0.0;
   {
+    // Courant number and amplitude of the south face of the first row
+    const T vv = D.v[at3(jw, i_c)];
+    const T rm = ROT ? T(0) : D.rmv[g.at2(jw, i_c)];
     Q mz, ma;
-    mface(i_c, mz, ma);
+    double mzd, mad;
+    if constexpr (ROT) {
+      mz = (Q)vv * dtrdy;
+      ma = Q(0.5) * (Q)vv;
+      mzd = (double)vv * dtrdy_d;
+      mad = 0.5 * (double)vv;
+    } else {
+      mz = (Q)vv * (Q)rm * dtrdy;
+      ma = Q(0.5) * (Q)vv * (Q)rm;
+      mzd = (double)vv * (double)rm * dtrdy_d;
+      mad = 0.5 * (double)vv * (double)rm;
+    }
 #pragma unroll
     for (int f = 0; f < NF; f++) {
       w[f][0] = F.wz[f][at3(jw, i_m2)];
       w[f][1] = F.wz[f][at3(jw, i_m1)];
       w[f][2] = F.wz[f][at3(jw, i_c)];
       w[f][3] = F.wz[f][at3(jw, i_p1)];
-      zs[f] = mflux(mz, ma, w[f][0], w[f][1], w[f][2], w[f][3]);
+      zs[f] = waf_mflux(mz, ma, w[f][0], w[f][1], w[f][2], w[f][3]);
+    }
+    if (ND) {
+      wd[0] = FD.wz[at3(jw, i_m2)];
+      wd[1] = FD.wz[at3(jw, i_m1)];
+      wd[2] = FD.wz[at3(jw, i_c)];
+      wd[3] = FD.wz[at3(jw, i_p1)];
+      zsd = waf_mflux(mzd, mad, wd[0], wd[1], wd[2], wd[3]);
     }
   }
   // field-independent loads of the row centred at ic, issued a row ahead:
   // v, fmz and rmv at its north neighbour (the south ones are carried from
   // the previous row), fmz east and west, u here and east, the map factors
-  struct RowIn {
-    Q vn, fn, fe, fw, uc, up, rvn, m0, m1, m2;
-  };
   auto row_in = [&](int ic, int in) {
-    RowIn o;
-    o.vn = (Q)D.v[at3(jw, in)];
-    o.fn = (Q)D.fmz[at3(jw, in)];
-    o.fe = (Q)D.fmz[at3(jp, ic)];
-    o.fw = (Q)D.fmz[at3(jm, ic)];
-    o.uc = (Q)D.u[at3(jw, ic)];
-    o.up = (Q)D.u[at3(jp, ic)];
-    o.rvn = (Q)D.rmv[g.at2(jw, in)];
+    HRow<T> o;
+    o.vn = D.v[at3(jw, in)];
+    o.fn = D.fmz[at3(jw, in)];
+    o.fe = D.fmz[at3(jp, ic)];
+    o.fw = D.fmz[at3(jm, ic)];
+    o.uc = D.u[at3(jw, ic)];
+    o.up = D.u[at3(jp, ic)];
+    o.rvn = D.rmv[g.at2(jw, in)];
     if constexpr (ROT) {
-      o.m0 = (Q)D.mx[g.at2(jw, ic)];
-      o.m1 = (Q)D.mu[g.at2(jw, ic)];
-      o.m2 = Q(0);
+      o.m0 = D.mx[g.at2(jw, ic)];
+      o.m1 = D.mu[g.at2(jw, ic)];
+      o.m2 = T(0);
     } else {
-      o.m0 = (Q)D.mx2[g.at2(jw, ic)];
-      o.m1 = (Q)D.rmu[g.at2(jw, ic)];
-      o.m2 = (Q)D.rmu[g.at2(jp, ic)];
+      o.m0 = D.mx2[g.at2(jw, ic)];
+      o.m1 = D.rmu[g.at2(jw, ic)];
+      o.m2 = D.rmu[g.at2(jp, ic)];
     }
     return o;
   };
   // carried south values of the current row
-  Q v_c = (Q)D.v[at3(jw, i_c)], rv_c = (Q)D.rmv[g.at2(jw, i_c)];
-  Q f_s = (Q)D.fmz[at3(jw, i_m1)], f_c = (Q)D.fmz[at3(jw, i_c)];
-  RowIn nx = row_in(i_c, i_p1);
+  T v_c = D.v[at3(jw, i_c)], rv_c = D.rmv[g.at2(jw, i_c)];
+  T f_s = D.fmz[at3(jw, i_m1)], f_c = D.fmz[at3(jw, i_c)];
+  HRow<T> nx = row_in(i_c, i_p1);
   // loads of the first row
   Q nw[NF], np[NF];
+  double nwd = 0.0, npd = 0.0;
 #pragma unroll
   for (int f = 0; f < NF; f++) {
     nw[f] = F.wz[f][at3(jw, i_p2)];
     np[f] = F.in[f][at3(jw, i_c)];
+  }
+  if (ND) {
+    nwd = FD.wz[at3(jw, i_p2)];
+    npd = FD.in[at3(jw, i_c)];
   }
   for (int r = 0; r < nrows; r++) {
     Q pp[NF];
@@ -1262,7 +1423,9 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, do
       w[f][4] = nw[f];
       pp[f] = np[f];
     }
-    const RowIn ci = nx;
+    const double ppd = npd;
+    if (ND) wd[4] = nwd;
+    const HRow<T> ci = nx;
     if (r + 1 < nrows) {  // uniform over the wave
       const int i_n = nxt(i_p2);
 #pragma unroll
@@ -1270,107 +1433,29 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, do
         nw[f] = F.wz[f][at3(jw, i_n)];
         np[f] = F.in[f][at3(jw, i_p1)];
       }
+      if (ND) {
+        nwd = FD.wz[at3(jw, i_n)];
+        npd = FD.in[at3(jw, i_p1)];
+      }
       nx = row_in(i_p1, i_p2);
     }
-    // field-independent coefficients of row i_c
-    Q mzn, man;  // north face (south face of row i_p1)
-    if constexpr (ROT) {
-      mzn = ci.vn * dtrdy;
-      man = Q(0.5) * ci.vn;
-    } else {
-      mzn = ci.vn * ci.rvn * dtrdy;
-      man = Q(0.5) * ci.vn * ci.rvn;
+    const HCoef<Q> cq = waf_hcoef<Q, ROT>(ci, v_c, rv_c, f_s, f_c, dtrdx, dtrdy);
+    const long xo = at3(ja <= nj ? ja : jw, i_c);
+#pragma unroll
+    for (int f = 0; f < NF; f++) {
+      const Q o = waf_hrow<Q, ROT>(w[f], zs[f], pp[f], cq, lm1, lm2, lp1);
+      if (store) F.out[f][xo] = o;
     }
-    Q pfs, pfn, pcd, pm2;
-    {
-      const Q f0 = f_c, fn = ci.fn, fs = f_s;
-      const Q vc = v_c, vp = ci.vn;
-      if constexpr (ROT) {
-        const Q mx = ci.m0;
-        const Q zhxvtn = dtrdy * ci.rvn * mx;
-        const Q zhxvts = dtrdy * rv_c * mx;
-        pfn = zhxvtn * Q(2) * f0 / (f0 + fn);
-        pfs = zhxvts * Q(2) * f0 / (f0 + fs);
-        pcd = vp * pfn - vc * pfs;
-        pm2 = Q(1);
-      } else {
-        pfn = dtrdy * Q(2) * f0 / (f0 + fn);
-        pfs = dtrdy * Q(2) * f0 / (f0 + fs);
-        pcd = vp * ci.rvn * pfn - vc * rv_c * pfs;
-        pm2 = ci.m0;
-      }
-    }
-    Q zz, za, zfw, zfe, zcd, zm2;
-    {
-      const Q uc = ci.uc, up = ci.up;
-      const Q f0 = f_c, fe = ci.fe, fw = ci.fw;
-      if constexpr (ROT) {
-        const Q mu = ci.m1;
-        zz = uc * mu * dtrdx;
-        za = Q(0.5) * uc;
-        const Q zcostx = dtrdx * mu;
-        zfe = zcostx * Q(2) * f0 / (f0 + fe);
-        zfw = zcostx * Q(2) * f0 / (f0 + fw);
-        zcd = up * zfe - uc * zfw;
-        zm2 = Q(1);
-      } else {
-        const Q rm = ci.m1;
-        zz = uc * rm * dtrdx;
-        za = Q(0.5) * uc * rm;
-        zfe = dtrdx * Q(2) * f0 / (f0 + fe);
-        zfw = dtrdx * Q(2) * f0 / (f0 + fw);
-        zcd = up * ci.m2 * zfe - uc * rm * zfw;
-        zm2 = ci.m0;
-      }
+    if (ND) {
+      const HCoef<double> cd = waf_hcoef<double, ROT>(ci, v_c, rv_c, f_s, f_c, dtrdx_d, dtrdy_d);
+      const double o = waf_hrow<double, ROT>(wd, zsd, ppd, cd, lm1, lm2, lp1);
+      if (store) FD.out[xo] = o;
     }
     // the north values become the next row's south ones
     v_c = ci.vn;
     rv_c = ci.rvn;
     f_s = f_c;
     f_c = ci.fn;
-    const long xo = at3(ja <= nj ? ja : jw, i_c);
-#pragma unroll
-    for (int f = 0; f < NF; f++) {
-      // meridional: flux at the north face, then p0 of row i_c
-      const Q zn = mflux(mzn, man, w[f][1], w[f][2], w[f][3], w[f][4]);
-      Q p0;
-      const Q zdv = pcd * pp[f];
-      if constexpr (ROT)
-        p0 = w[f][2] + zs[f] * pfs - zn * pfn + zdv;
-      else
-        p0 = w[f][2] + pm2 * (zs[f] * pfs - zn * pfn + zdv);
-      // zonal: flux at the west face of the lane's column, then the result
-      const Q pm = __shfl(p0, lm1, 64), pmm = __shfl(p0, lm2, 64), ppl = __shfl(p0, lp1, 64);
-      Q is, a, b;
-      if (zz > Q(0)) {
-        is = Q(1);
-        a = pm;
-        b = pmm;
-      } else {
-        is = Q(-1);
-        a = ppl;
-        b = p0;
-      }
-      const Q zzden = sgn_floor(p0 - pm);
-      const Q rq = (a - b) / zzden;
-      const Q bb = waf_b(rq);
-      const Q zphi = is + zz * bb - is * bb;
-      const Q zw = za * ((Q(1) + zphi) * pm + (Q(1) - zphi) * p0);
-      const Q ze = __shfl(zw, lp1, 64);
-      const Q zdu = zcd * pp[f];
-      Q o;
-      if constexpr (ROT)
-        o = p0 + zw * zfw - ze * zfe + zdu;
-      else
-        o = p0 + zm2 * (zw * zfw - ze * zfe + zdu);
-      if (store) F.out[f][xo] = o;
-      // slide the window
-      w[f][0] = w[f][1];
-      w[f][1] = w[f][2];
-      w[f][2] = w[f][3];
-      w[f][3] = w[f][4];
-      zs[f] = zn;
-    }
     i_m2 = i_m1;
     i_m1 = i_c;
     i_c = i_p1;
@@ -2764,8 +2849,11 @@ struct Core : CoreBase {
 
   // wafone on a group of fields of one storage type: k_wafv then k_wafh; each
   // field's result buffer then takes the field's place.
-  template <typename Q, int NF>
-  void waf_group(Q **fld[], Q **outs, Q **wzs, double dta) {
+  // One group: NF fields of type Q (in *fld[f], scratch wzs[f], results into
+  // outs[f], then swapped into place) and, with ND = 1, the FP64 rider
+  // *dfld (scratch dwz, results into *dout, swapped likewise).
+  template <typename Q, int NF, int ND>
+  void waf_group(Q **fld[], Q **outs, Q **wzs, double **dfld, double **dout, double *dwz, double dta) {
     Par P = par();
     FieldSet<Q> F;
     for (int f = 0; f < MAXF; f++) F.in[f] = nullptr, F.wz[f] = nullptr, F.out[f] = nullptr;
@@ -2774,37 +2862,58 @@ struct Core : CoreBase {
       F.wz[f] = wzs[f];
       F.out[f] = outs[f];
     }
+    FieldD FD{nullptr, nullptr, nullptr};
+    if (ND) FD = FieldD{*dfld, dwz, *dout};
     const double q = sizeof(Q) == sizeof(double) ? 1.0 : 0.0;
-    const double sT = sizeof(T) == sizeof(double) ? 1.0 : 0.0;
-    // vertical: s, fmz, fmzf + NF fields -> NF wz
-    launch(KK_WAF_V, fbytes(3, 0) + (q ? fbytes(0, NF) : fbytes(NF, 0)), k_wafv<T, Q, NF>, col_grid(),
-           blk(), D, P, F, dta);
-    // horizontal: u, v, fmz + NF x (wz, field) -> NF results
+    // vertical: s, fmz, fmzf + the fields -> wz (the rider on its own)
+    launch(KK_WAF_V, fbytes(3, 0) + (q ? fbytes(0, NF) : fbytes(NF, 0)), k_wafv<T, Q, NF>, col_grid(), blk(), D, P,
+           F, dta);
+    if (ND) {
+      FieldSet<double> Fd;
+      for (int f = 0; f < MAXF; f++) Fd.in[f] = nullptr, Fd.wz[f] = nullptr, Fd.out[f] = nullptr;
+      Fd.in[0] = FD.in;
+      Fd.wz[0] = FD.wz;
+      launch(KK_WAF_V, fbytes(0, 1), k_wafv<T, double, 1>, col_grid(), blk(), D, P, Fd, dta);
+    }
+    // horizontal: u, v, fmz + the fields' wz and starting values -> results
     const int ntj = (g.nj + HJ - 1) / HJ, nti = (g.ni + HR - 1) / HR;
     const long total = ((long)ntj * nti * g.nk + 3) / 4;  // blocks of four strips
+    const double hb = (q ? fbytes(0, NF) : fbytes(NF, 0)) + fbytes(0, ND);
     if (prm.rotllr)
-      launch(KK_WAF_H, q ? fbytes(0, NF) : fbytes(NF, 0), k_wafh<T, Q, NF, true>,
-             dim3((unsigned)round_up(total, 8)), dim3(256), D, P, F, dta, ntj, nti);
+      launch(KK_WAF_H, hb, k_wafh<T, Q, NF, ND, true>, dim3((unsigned)round_up(total, 8)), dim3(256), D, P, F, FD,
+             dta, ntj, nti);
     else
-      launch(KK_WAF_H, q ? fbytes(0, NF) : fbytes(NF, 0), k_wafh<T, Q, NF, false>,
-             dim3((unsigned)round_up(total, 8)), dim3(256), D, P, F, dta, ntj, nti);
-    (void)sT;
+      launch(KK_WAF_H, hb, k_wafh<T, Q, NF, ND, false>, dim3((unsigned)round_up(total, 8)), dim3(256), D, P, F, FD,
+             dta, ntj, nti);
     for (int f = 0; f < NF; f++) std::swap(*fld[f], outs[f]);
+    if (ND) std::swap(*dfld, *dout);
   }
-
+  // The n fields in groups of at most MAXF; the FP64 rider, if any, goes
+  // with the last group.
   template <typename Q>
-  void waf_groups(Q **fld[], int n, Q **outs, Q **wzs, double dta) {
+  void waf_groups(Q **fld[], int n, Q **outs, Q **wzs, double **dfld, double **dout, double *dwz, double dta) {
     for (int b = 0; b < n; b += MAXF) {
-      int m = n - b < MAXF ? n - b : MAXF;
+      const int m = n - b < MAXF ? n - b : MAXF;
+      const bool rider = dfld && b + m == n;
+#define MLC_WAF_GROUP(NF)                                                   \
+  if constexpr (sizeof(Q) != sizeof(double)) {                              \
+    if (rider) {                                                            \
+      waf_group<Q, NF, 1>(fld + b, outs, wzs, dfld, dout, dwz, dta);       \
+      break;                                                                \
+    }                                                                       \
+  }                                                                         \
+  waf_group<Q, NF, 0>(fld + b, outs, wzs, nullptr, nullptr, nullptr, dta);
       switch (m) {
-        case 1: waf_group<Q, 1>(fld + b, outs, wzs, dta); break;
-        case 2: waf_group<Q, 2>(fld + b, outs, wzs, dta); break;
-        case 3: waf_group<Q, 3>(fld + b, outs, wzs, dta); break;
-        case 4: waf_group<Q, 4>(fld + b, outs, wzs, dta); break;
-        default: waf_group<Q, 5>(fld + b, outs, wzs, dta); break;
+        case 1: MLC_WAF_GROUP(1) break;
+        case 2: MLC_WAF_GROUP(2) break;
+        case 3: MLC_WAF_GROUP(3) break;
+        case 4: MLC_WAF_GROUP(4) break;
+        default: MLC_WAF_GROUP(5) break;
       }
+#undef MLC_WAF_GROUP
     }
   }
+
 
   void advection(double dta) {
     halo(D.u, g.nk, 2);
@@ -2821,12 +2930,11 @@ struct Core : CoreBase {
       list[n++] = &D.wx;
       list[n++] = &D.qv;
       for (int s = prm.iqfrst; s <= dims.nqx; s++) list[n++] = qm[s - 1];
-      waf_groups<T>(list, n, gout, gwz, dta);
       if constexpr (sizeof(T) != sizeof(double)) {
-        double **dl[1] = {&D.pai};
-        double *dout[1] = {D.smd}, *dwz[1] = {D.wz};
-        waf_groups<double>(dl, 1, dout, dwz, dta);
-        D.smd = dout[0];
+        // the FP64 Exner function rides with the last group of FP32 fields
+        waf_groups<T>(list, n, gout, gwz, &D.pai, &D.smd, D.wz, dta);
+      } else {
+        waf_groups<T>(list, n, gout, gwz, nullptr, nullptr, nullptr, dta);
       }
       halo(D.ux, g.nk, 2);
       halo(D.vx, g.nk, 3);
