@@ -8,7 +8,8 @@ core (FP64 Exner function, hydrostatic residuals and column solve) and the
 FP32 core with the learned acoustic corrector (one long acoustic substep plus
 a BF16 MFMA network increment, then the last ordinary substep; the network is
 trained on the GPU from ordinary steps of the same run before timing), with
-each variant's relative L2 error against the FP64 core after the same steps.
+each variant's relative L2 and max-abs error against the FP64 core after the
+same steps (dynamical_model_amd.errgrowth follows them over long runs).
 
 Workload (default): configuration 4, 2048 x 2048 x 60, synthetic initial state
 (standard atmosphere, 20 seeded Gaussian hills, jet, RH 60 %, theta noise)
@@ -228,6 +229,8 @@ def main() -> int:
 
     errors = {tag: {f: rel_l2(finals[tag][f], finals["fp64"][f]) for f in PROGNOSTIC}
               for tag in results if tag != "fp64"}
+    maxabs = {tag: {f: float(np.max(np.abs(finals[tag][f] - finals["fp64"][f]))) for f in PROGNOSTIC}
+              for tag in results if tag != "fp64"}
     del finals
 
     chosen_tag = "learned" if "learned" in results else ("fp32" if "fp32" in results else "fp64")
@@ -278,6 +281,7 @@ def main() -> int:
         "fp64": {"value": results["fp64"]["value"], "ms_per_step": results["fp64"]["ms_per_step"], "dtype": "f64"},
         "speedup_vs_fp64": {t: r["value"] / results["fp64"]["value"] for t, r in results.items()},
         "rel_l2_vs_fp64": errors,
+        "maxabs_vs_fp64": maxabs,
         "rel_l2_after_steps": args.lead + args.warmup + args.steps + 1,
         "roofline": roofline,
         "stencil_roofline": stencil,
