@@ -135,16 +135,17 @@ def dominant(prof: dict) -> tuple[str, float, float, int]:
     return name, ms, b, n
 
 
-def cpu_baseline(jx: int, iy: int, kz: int, steps: int):
-    """The CPU oracle (a single-threaded C restatement of the reference step)
-    on a bounded sample of the same workload."""
+def cpu_baseline(jx: int, iy: int, kz: int, steps: int, threaded: bool = False):
+    """The CPU oracle (a C restatement of the reference step; serial, or the
+    same source with loop-level OpenMP threads on all host cores, which
+    computes the same bits) on a bounded sample of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import dynamical_model_amd as dm
     import oracle_lib
 
     case = dm.case_config(4, jx=jx, iy=iy, kz=kz, hill_hwmax=min(40.0, jx / 6.0))
     hs = dm.build_case(case)
-    o = oracle_lib.oracle_from_host(hs, dm.case_params(case))
+    o = oracle_lib.oracle_from_host(hs, dm.case_params(case), threaded=threaded)
     t0 = time.perf_counter()
     for n in range(1, steps + 1):
         o.moloch(n, False)
@@ -165,7 +166,10 @@ def main() -> int:
     ap.add_argument("--jx", type=int, default=0)
     ap.add_argument("--iy", type=int, default=0)
     ap.add_argument("--kz", type=int, default=0)
-    ap.add_argument("--cpu-sample", default="128x128x10", help="JXxIYxSTEPS of the CPU oracle sample, or 'none'")
+    ap.add_argument("--cpu-sample", default="128x128x10",
+                    help="JXxIYxSTEPS of the serial CPU oracle sample, or 'none'")
+    ap.add_argument("--cpu-sample-threaded", default="256x256x10",
+                    help="JXxIYxSTEPS of the all-core CPU oracle sample, or 'none'")
     ap.add_argument("--lead", type=int, default=10,
                     help="ordinary steps before warm-up (the corrector's training run)")
     ap.add_argument("--train-iters", type=int, default=6000, help="Adam steps of the corrector")
@@ -245,13 +249,25 @@ def main() -> int:
     mlp = roofline_of(chosen["_prof"], "lac_mlp", traffic) if "lac_mlp" in chosen["_prof"] else None
 
     cpu = None
+    serial = None
     if args.cpu_sample != "none":
         sj, si, ss = (int(x) for x in args.cpu_sample.split("x"))
         v, dt = cpu_baseline(sj, si, kz, ss)
-        cpu = {"value": v, "unit": "column-steps/s", "cores": 1, "kind": "port",
-               "sample": f"CPU oracle (C restatement of nompi/mod_moloch.F90, FP64, 1 thread), configuration-"
-                         f"{args.config} physics on {sj}x{si}x{kz} for {ss} steps ({dt:.1f} s)"}
-        log(f"cpu baseline {v:.4g} column-steps/s ({dt:.1f} s)")
+        serial = {"value": v, "unit": "column-steps/s", "cores": 1, "kind": "port",
+                  "sample": f"CPU oracle (C restatement of nompi/mod_moloch.F90, FP64, 1 thread), configuration-"
+                            f"{args.config} physics on {sj}x{si}x{kz} for {ss} steps ({dt:.1f} s)"}
+        log(f"cpu baseline, 1 core: {v:.4g} column-steps/s ({dt:.1f} s)")
+        cpu = serial
+    if args.cpu_sample_threaded != "none":
+        sj, si, ss = (int(x) for x in args.cpu_sample_threaded.split("x"))
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        v, dt = cpu_baseline(sj, si, kz, ss, threaded=True)
+        cpu = {"value": v, "unit": "column-steps/s", "cores": threads, "kind": "port",
+               "sample": f"CPU oracle (C restatement of nompi/mod_moloch.F90, FP64, loop-level OpenMP on "
+                         f"{threads} threads, bit-identical to the serial build), configuration-{args.config} "
+                         f"physics on {sj}x{si}x{kz} for {ss} steps ({dt:.1f} s)",
+               "one_core": serial}
+        log(f"cpu baseline, {threads} threads: {v:.4g} column-steps/s ({dt:.1f} s)")
 
     for r in results.values():
         r.pop("_prof")

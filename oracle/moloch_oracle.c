@@ -36,6 +36,22 @@
  */
 #include "moloch_oracle.h"
 
+/* Loop-level threads for the CPU baseline build (-fopenmp, the library
+ * libmoloch_oracle_omp.so).  Every marked loop nest writes only elements of
+ * its own (j, i, k), or of its own column, and reads nothing the same nest
+ * writes elsewhere, so the threaded build computes the same bits as the
+ * serial one (tests/test_oracle_pins.py checks it).  Without -fopenmp the
+ * marks are empty. */
+#ifdef _OPENMP
+#define ORA_PAR_IJ _Pragma("omp parallel for schedule(static) private(i, j)")
+#define ORA_PAR_J _Pragma("omp parallel for schedule(static) private(j)")
+#define ORA_PAR_COLUMNS _Pragma("omp parallel for schedule(static) private(j, k, zwexpl, zu, zd, wcol)")
+#else
+#define ORA_PAR_IJ
+#define ORA_PAR_J
+#define ORA_PAR_COLUMNS
+#endif
+
 #include <math.h>
 #include <stddef.h>
 
@@ -247,12 +263,15 @@ static void halo_i2(const ora_state *S, double *f) {
  * :1313-1325 */
 void ora_reset_tendencies(ora_state *S) {
   int nj = S->nj, ni = S->ni, nk = S->nk, j, i, k;
+  ORA_PAR_IJ
   for (k = 1; k <= nk + 1; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++) F3(S->s, j, i, k) = 0.0;
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 0; i <= ni + 1; i++)
       for (j = 0; j <= nj + 1; j++) F3(S->zdiv2, j, i, k) = 0.0;
+  ORA_PAR_IJ
   for (k = 1; k <= nk + 1; k++)
     for (i = 0; i <= ni + 1; i++)
       for (j = 0; j <= nj + 1; j++) F3(S->deltaw, j, i, k) = 0.0;
@@ -263,6 +282,7 @@ void ora_reset_tendencies(ora_state *S) {
 void ora_prologue(ora_state *S) {
   int nj = S->nj, ni = S->ni, nk = S->nk, j, i, k;
   double *qv = QX(1), *qc = QX(2), *qi = QX(3), *qr = QX(4), *qs = QX(5);
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++) {
@@ -270,6 +290,7 @@ void ora_prologue(ora_state *S) {
         S->p[x] = pow(S->pai[x], CPOVR) * P00;
         S->qsat[x] = ora_pfwsat(S->t[x], S->p[x]);
       }
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++) {
@@ -277,6 +298,7 @@ void ora_prologue(ora_state *S) {
         S->tvirt[x] = S->t[x] * (1.0 + EP1 * qv[x] - qc[x] - qi[x] - qr[x] - qs[x]);
       }
   if (S->do_fulleq) {
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++) {
@@ -285,6 +307,7 @@ void ora_prologue(ora_state *S) {
           S->qwitot[x] = qi[x] + qs[x];
         }
   }
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++) {
@@ -292,11 +315,13 @@ void ora_prologue(ora_state *S) {
         S->tetav[x] = S->tvirt[x] / S->pai[x];
       }
   if (S->do_filterpai) {
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++) F3(S->pf, j, i, k) = F3(S->pai, j, i, k);
   }
   if (S->do_fulleq && S->do_filtertheta) {
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++) F3(S->tf, j, i, k) = F3(S->tetav, j, i, k);
@@ -309,11 +334,13 @@ void ora_prologue(ora_state *S) {
  */
 static void smooth5(ora_state *S, double *f, int k, double coef) {
   int nj = S->nj, ni = S->ni, j, i;
+  ORA_PAR_J
   for (i = 1; i <= ni; i++)
     for (j = 1; j <= nj; j++)
       F2(S->p2d, j, i) = 0.125 * (F3(f, j - 1, i, k) + F3(f, j + 1, i, k) +
                                   F3(f, j, i - 1, k) + F3(f, j, i + 1, k)) -
                          0.5 * F3(f, j, i, k);
+  ORA_PAR_J
   for (i = 1; i <= ni; i++)
     for (j = 1; j <= nj; j++) F3(f, j, i, k) = F3(f, j, i, k) + coef * F2(S->p2d, j, i);
 }
@@ -325,11 +352,13 @@ void ora_divergence_filter(ora_state *S) {
     /* mo_anu2 * xknu(k) * p2d */
     int nj = S->nj, ni = S->ni, j, i;
     double *f = S->zdiv2;
+    ORA_PAR_J
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++)
         F2(S->p2d, j, i) = 0.125 * (F3(f, j - 1, i, k) + F3(f, j + 1, i, k) +
                                     F3(f, j, i - 1, k) + F3(f, j, i + 1, k)) -
                            0.5 * F3(f, j, i, k);
+    ORA_PAR_J
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++)
         F3(f, j, i, k) = F3(f, j, i, k) + S->anu2 * S->xknu[k - 1] * F2(S->p2d, j, i);
@@ -355,6 +384,7 @@ void ora_divdamp(ora_state *S, double dts) {
   double ddamp;
   halo_faces1(S, S->zdiv2, nk);
   ddamp = 0.125 * (S->dx / dts);
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++)
@@ -362,6 +392,7 @@ void ora_divdamp(ora_state *S, double dts) {
                             S->xknu[k - 1] * ddamp * F2(S->mu, j, i) *
                                 (F3(S->zdiv2, j, i, k) - F3(S->zdiv2, j - 1, i, k));
   if (S->rotllr) {
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++)
@@ -369,6 +400,7 @@ void ora_divdamp(ora_state *S, double dts) {
                               S->xknu[k - 1] * ddamp *
                                   (F3(S->zdiv2, j, i, k) - F3(S->zdiv2, j, i - 1, k));
   } else {
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++)
@@ -411,6 +443,7 @@ void ora_sound(ora_state *S, double dts) {
   double *qv = QX(1), *qc = QX(2);
   double zwexpl[1026], zu[1026], zd[1026], wcol[1026];
 
+  ORA_PAR_J
   for (i = 1; i <= ni; i++)
     for (j = 1; j <= nj; j++) F3(S->wwkw, j, i, nk + 1) = 0.0;
 
@@ -421,16 +454,19 @@ void ora_sound(ora_state *S, double dts) {
     halo_corners1(S, v); /* :522-529 */
 
     /* surface boundary condition of w (:533-545) and s (:547-549) */
+    ORA_PAR_J
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++) {
         double zuh = F3(u, j, i, nk) * F2(S->hx, j, i) + F3(u, j + 1, i, nk) * F2(S->hx, j + 1, i);
         double zvh = F3(v, j, i, nk) * F2(S->hy, j, i) + F3(v, j, i + 1, nk) * F2(S->hy, j, i + 1);
         F3(w, j, i, nk + 1) = 0.5 * (zuh + zvh);
       }
+    ORA_PAR_J
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++) F3(s, j, i, nk + 1) = -F3(w, j, i, nk + 1);
 
     /* generalised vertical velocity, Equation 10 (:553-570) */
+    ORA_PAR_IJ
     for (k = 2; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++) {
@@ -442,6 +478,7 @@ void ora_sound(ora_state *S, double dts) {
         }
 
     /* horizontal divergence, Equation 16 (:575-624) */
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++) {
@@ -469,6 +506,7 @@ void ora_sound(ora_state *S, double dts) {
     if (S->do_divdamp) ora_divdamp(S, dts); /* :626 */
 
     /* vertical part of the divergence (:627-630) */
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++)
@@ -476,6 +514,7 @@ void ora_sound(ora_state *S, double dts) {
                                F3(fmz, j, i, k) * dtrdz * (F3(s, j, i, k) - F3(s, j, i, k + 1));
 
     /* implicit w, Equation 19 (:634-686), one column at a time */
+    ORA_PAR_COLUMNS
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++) {
         for (k = nk; k >= 2; k--) {
@@ -508,6 +547,7 @@ void ora_sound(ora_state *S, double dts) {
       }
 
     /* new Exner function, Equation 19 (:690-693) */
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++)
@@ -515,6 +555,7 @@ void ora_sound(ora_state *S, double dts) {
                                dtrdz * F3(fmz, j, i, k) * (F3(w, j, i, k) - F3(w, j, i, k + 1));
 
     if (S->do_fulleq) { /* :695-712 */
+      ORA_PAR_IJ
       for (k = 1; k <= nk; k++)
         for (i = 1; i <= ni; i++)
           for (j = 1; j <= nj; j++) {
@@ -533,6 +574,7 @@ void ora_sound(ora_state *S, double dts) {
       halo_faces1(S, pai, nk);
       halo_faces1(S, deltaw, nk + 1);
     }
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++) /* :726-728 */
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++)
@@ -543,6 +585,7 @@ void ora_sound(ora_state *S, double dts) {
     }
 
     /* snapshots including halos (:730-735) */
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 0; i <= ni + 1; i++)
         for (j = 0; j <= nj + 1; j++) {
@@ -551,6 +594,7 @@ void ora_sound(ora_state *S, double dts) {
         }
 
     /* horizontal momentum, Equations 17 and 18 (:737-838) */
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++) {
@@ -564,6 +608,7 @@ void ora_sound(ora_state *S, double dts) {
           F3(u, j, i, k) = F3(u, j, i, k) + zcor1u - zfz * F2(S->hx, j, i) * S->gzitakh[k - 1] -
                            zcx * zrom1u * (F3(pai, j, i, k) - F3(pai, j - 1, i, k));
         }
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++) {
@@ -580,10 +625,12 @@ void ora_sound(ora_state *S, double dts) {
   } /* sound loop */
 
   /* complete Equation 10 (:843-849) */
+  ORA_PAR_IJ
   for (k = 2; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++)
         F3(s, j, i, k) = (F3(w, j, i, k) + F3(s, j, i, k)) * F3(fmzf, j, i, k);
+  ORA_PAR_J
   for (i = 1; i <= ni; i++)
     for (j = 1; j <= nj; j++) {
       F3(s, j, i, 1) = 0.0;
@@ -614,11 +661,13 @@ void ora_wafone(ora_state *S, double *pp, double dta) {
   /* vertical advection (:951-1073): pass 0 reads pp, pass 1 reads wz */
   for (pass = 0; pass < (S->do_vadvtwice ? 2 : 1); pass++) {
     double *q = pass == 0 ? pp : wz;
+    ORA_PAR_J
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++) {
         F3(wfw, j, i, 1) = 0.0;
         F3(wfw, j, i, nk + 1) = 0.0;
       }
+    ORA_PAR_IJ
     for (k = 1; k <= nk - 1; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++) {
@@ -642,6 +691,7 @@ void ora_wafone(ora_state *S, double *pp, double dta) {
           F3(wfw, j, i, k + 1) = 0.5 * F3(s, j, i, k + 1) *
                                  ((1.0 + zphi) * F3(q, j, i, k + 1) + (1.0 - zphi) * F3(q, j, i, k));
         }
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++) {
@@ -656,6 +706,7 @@ void ora_wafone(ora_state *S, double *pp, double dta) {
   halo_i2(S, wz); /* :1075-1078 */
 
   /* meridional advection (:1082-1131 / :1190-1240) */
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni + 1; i++)
       for (j = 1; j <= nj; j++) {
@@ -680,6 +731,7 @@ void ora_wafone(ora_state *S, double *pp, double dta) {
           F3(zpby, j, i, k) = 0.5 * vv * F2(S->rmv, j, i) *
                               ((1.0 + zphi) * F3(wz, j, i - 1, k) + (1.0 - zphi) * F3(wz, j, i, k));
       }
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++) {
@@ -706,6 +758,7 @@ void ora_wafone(ora_state *S, double *pp, double dta) {
   halo_j2(S, p0); /* :1133-1136 */
 
   /* zonal advection (:1138-1187 / :1247-1296) */
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj + 1; j++) {
@@ -730,6 +783,7 @@ void ora_wafone(ora_state *S, double *pp, double dta) {
           F3(zpbw, j, i, k) = 0.5 * uu * F2(S->rmu, j, i) *
                               ((1.0 + zphi) * F3(p0, j - 1, i, k) + (1.0 - zphi) * F3(p0, j, i, k));
       }
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++) {
@@ -758,11 +812,13 @@ void ora_wafone(ora_state *S, double *pp, double dta) {
 void ora_wstagtox(ora_state *S) {
   int nj = S->nj, ni = S->ni, nk = S->nk, j, i, k;
   double *w = S->w, *wx = S->wx;
+  ORA_PAR_IJ
   for (k = 2; k <= nk - 1; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++)
         F3(wx, j, i, k) = 0.5625 * (F3(w, j, i, k + 1) + F3(w, j, i, k)) -
                           0.0625 * (F3(w, j, i, k + 2) + F3(w, j, i, k - 1));
+  ORA_PAR_J
   for (i = 1; i <= ni; i++)
     for (j = 1; j <= nj; j++) {
       F3(wx, j, i, 1) = 0.5 * (F3(w, j, i, 2) + F3(w, j, i, 1));
@@ -774,11 +830,13 @@ void ora_wstagtox(ora_state *S) {
 void ora_xtowstag(ora_state *S) {
   int nj = S->nj, ni = S->ni, nk = S->nk, j, i, k;
   double *w = S->w, *wx = S->wx;
+  ORA_PAR_IJ
   for (k = 3; k <= nk - 1; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++)
         F3(w, j, i, k) = 0.5625 * (F3(wx, j, i, k) + F3(wx, j, i, k - 1)) -
                          0.0625 * (F3(wx, j, i, k + 1) + F3(wx, j, i, k - 2));
+  ORA_PAR_J
   for (i = 1; i <= ni; i++)
     for (j = 1; j <= nj; j++) {
       F3(w, j, i, 2) = 0.5 * (F3(wx, j, i, 2) + F3(wx, j, i, 1));
@@ -792,11 +850,13 @@ void ora_xtouvstag(ora_state *S) {
   double *ux = S->ux, *vx = S->vx, *u = S->u, *v = S->v;
   halo_j2(S, ux);
   halo_i2(S, vx);
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++)
         F3(u, j, i, k) = 0.5625 * (F3(ux, j, i, k) + F3(ux, j - 1, i, k)) -
                          0.0625 * (F3(ux, j + 1, i, k) + F3(ux, j - 2, i, k));
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++)
@@ -810,11 +870,13 @@ void ora_uvstagtox(ora_state *S) {
   double *ux = S->ux, *vx = S->vx, *u = S->u, *v = S->v;
   halo_j2(S, u);
   halo_i2(S, v);
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++)
         F3(ux, j, i, k) = 0.5625 * (F3(u, j + 1, i, k) + F3(u, j, i, k)) -
                           0.0625 * (F3(u, j + 2, i, k) + F3(u, j - 1, i, k));
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++)
@@ -844,18 +906,21 @@ void ora_advection(ora_state *S, double dta) {
 void ora_epilogue(ora_state *S) {
   int nj = S->nj, ni = S->ni, nk = S->nk, j, i, k;
   double *qv = QX(1), *qc = QX(2), *qi = QX(3), *qr = QX(4), *qs = QX(5);
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++) {
         long x = I3(j, i, k);
         S->tvirt[x] = S->tetav[x] * S->pai[x];
       }
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++) {
         long x = I3(j, i, k);
         S->t[x] = S->tvirt[x] / (1.0 + EP1 * qv[x] - qc[x] - qi[x] - qr[x] - qs[x]);
       }
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++) {
@@ -863,6 +928,7 @@ void ora_epilogue(ora_state *S) {
         S->p[x] = pow(S->pai[x], CPOVR) * P00;
         S->rho[x] = S->p[x] / (RGAS * S->t[x]);
       }
+  ORA_PAR_J
   for (i = 1; i <= ni; i++)
     for (j = 1; j <= nj; j++) {
       double zdgz = F3(S->zeta, j, i, nk) * K_EGRAV;
@@ -873,6 +939,7 @@ void ora_epilogue(ora_state *S) {
       tv = F3(S->tvirt, j, i, nk) - 0.5 * F3(S->zeta, j, i, nk) * lrt;
       F2(S->ps, j, i) = F3(S->p, j, i, nk) * exp(zdgz / (RGAS * tv));
     }
+  ORA_PAR_IJ
   for (k = 1; k <= nk; k++)
     for (i = 1; i <= ni; i++)
       for (j = 1; j <= nj; j++) {
@@ -930,20 +997,24 @@ void ora_moloch(ora_state *S, int istep, int lprint) {
   }
 
   if (S->do_filterpai) { /* :333-341 */
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++) F3(S->pai, j, i, k) = F3(S->pai, j, i, k) - F3(S->pf, j, i, k);
     ora_filtpai(S);
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++) F3(S->pai, j, i, k) = F3(S->pai, j, i, k) + F3(S->pf, j, i, k);
   }
   if (S->do_fulleq && S->do_filtertheta) { /* :343-353 */
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++)
           F3(S->tetav, j, i, k) = F3(S->tetav, j, i, k) - F3(S->tf, j, i, k);
     ora_filttheta(S);
+    ORA_PAR_IJ
     for (k = 1; k <= nk; k++)
       for (i = 1; i <= ni; i++)
         for (j = 1; j <= nj; j++)

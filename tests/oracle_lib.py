@@ -15,6 +15,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "libmoloch_oracle.so")
+# the same source built with loop-level OpenMP threads (the all-core CPU
+# baseline); bit-identical to the serial build
+ORACLE_LIB_OMP = os.path.join(ORACLE_DIR, "libmoloch_oracle_omp.so")
 
 _D = C.POINTER(C.c_double)
 
@@ -50,19 +53,19 @@ class OraConstants(C.Structure):
         "wlhv", "tzero", "lrate", "eomeg2", "degrad", "mathpi", "regrav", "earthrad", "raddeg")]
 
 
-_lib = None
+_libs = {}
 
 
 def build_oracle() -> None:
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
 
-def lib() -> C.CDLL:
-    global _lib
-    if _lib is None:
-        if not os.path.exists(ORACLE_LIB):
+def lib(threaded: bool = False) -> C.CDLL:
+    if threaded not in _libs:
+        path = ORACLE_LIB_OMP if threaded else ORACLE_LIB
+        if not os.path.exists(path):
             build_oracle()
-        L = C.CDLL(ORACLE_LIB)
+        L = C.CDLL(path)
         P = C.POINTER(OraState)
         for name, args in {
             "ora_init_moloch": [P], "ora_reset_tendencies": [P], "ora_prologue": [P],
@@ -83,8 +86,8 @@ def lib() -> C.CDLL:
             f.restype = C.c_double
         L.ora_gzita.argtypes = [C.c_double, C.c_double, C.c_double]
         L.ora_gzita.restype = C.c_double
-        _lib = L
-    return _lib
+        _libs[threaded] = L
+    return _libs[threaded]
 
 
 def constants() -> dict:
@@ -109,8 +112,9 @@ class Oracle:
     }
     QX = ["qv", "qc", "qi", "qr", "qs"]
 
-    def __init__(self, jx, iy, kz, params, levels=None):
+    def __init__(self, jx, iy, kz, params, levels=None, threaded=False):
         self.jx, self.iy, self.kz = jx, iy, kz
+        self.L = lib(threaded)
         self.a = {}
         plane = (iy + 4, jx + 4)
         for n in _PTRS_1D:
@@ -160,20 +164,20 @@ class Oracle:
         return self.a[self.STATE_MAP[name]]
 
     def init_moloch(self):
-        lib().ora_init_moloch(C.byref(self.S))
+        self.L.ora_init_moloch(C.byref(self.S))
 
     def moloch(self, istep=1, lprint=False):
-        lib().ora_moloch(C.byref(self.S), int(istep), 1 if lprint else 0)
+        self.L.ora_moloch(C.byref(self.S), int(istep), 1 if lprint else 0)
 
     def call(self, name, *args):
-        return getattr(lib(), name)(C.byref(self.S), *args)
+        return getattr(self.L, name)(C.byref(self.S), *args)
 
     def wafone(self, arr: np.ndarray, dta: float):
-        lib().ora_wafone(C.byref(self.S), _ptr(arr), dta)
+        self.L.ora_wafone(C.byref(self.S), _ptr(arr), dta)
 
 
-def oracle_from_host(hs, params) -> Oracle:
-    o = Oracle(hs.jx, hs.iy, hs.kz, params, hs.levels)
+def oracle_from_host(hs, params, threaded: bool = False) -> Oracle:
+    o = Oracle(hs.jx, hs.iy, hs.kz, params, hs.levels, threaded)
     o.load_fields(hs.fields)
     o.init_moloch()
     return o

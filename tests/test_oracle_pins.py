@@ -474,6 +474,23 @@ def _shift_state(hs, dj, di):
     return out
 
 
+@pytest.mark.parametrize("over", [{}, {"halo_mpi": 1}, {"divfilter": 1}, {"rotllr": 0}])
+def test_threaded_oracle_matches_serial(over):
+    """The all-core CPU baseline (the oracle source built with loop-level
+    OpenMP threads) computes the serial oracle's bits: every threaded loop
+    nest writes only its own points."""
+    c = small_config2(61, 37, 17)
+    hs = dm.build_case(c)
+    p = dm.case_params(c, **over)
+    a = oracle_lib.oracle_from_host(hs, p)
+    b = oracle_lib.oracle_from_host(hs, p, threaded=True)
+    for n in range(1, 3):
+        a.moloch(n, True)
+        b.moloch(n, True)
+    for f in ("u", "v", "w", "pai", "tetav", "qv", "t", "ps"):
+        np.testing.assert_array_equal(a.field(f), b.field(f), err_msg=f)
+
+
 def test_step_is_translation_equivariant_with_fresh_halos():
     """With the MPI tree's halo order and f = 0 every halo that is read is
     fresh, so the periodic step commutes with shifts bit for bit: any wrong
