@@ -1966,6 +1966,180 @@ __global__ void __launch_bounds__(FT) k_mlp_dense(FusedArgs a) {
     }
 }
 
+// Layers 2 and 3 in one kernel when the hidden width is TH = 512 and layer 3
+// has at most 256 outputs.  A block owns TM = 128 rows.  Layer 2
+// (h2 = relu(h1 W2 + b2), 128 x 512) streams its operands through LDS as
+// k_mlp_dense does: stages of two k-steps by global_load_lds, three buffers,
+// two stages ahead, one block barrier per stage; eight waves of 64 rows x
+// 128 outputs.  Its bf16 results stay in LDS (the stage buffers lie inside
+// that tile, which is written only after the last stage has been read), and
+// layer 3 (y = h2 W3 + b3) takes its A operand from there and its weight
+// fragments from global memory, four k-steps ahead; eight waves of 64 rows x
+// 64 outputs.  Every accumulator sees its MFMAs in k_mlp_dense's k order, so
+// the results are bit-identical to the two launches; h1 is read once and h2
+// never reaches HBM.
+constexpr int TM = 128, TH = 512, TKS = TH / 16;     // rows, hidden width, k-steps
+constexpr int TSA = TM * 32;                          // bf16 of one A stage
+constexpr int TSB = 2 * (TH / 32) * 64;               // 16-byte fragment lanes of one B stage
+constexpr int TBUF = TSA + TSB * 8;                   // bf16 of one stage buffer
+static_assert(3 * TBUF <= TM * TH, "stage buffers inside the h2 tile");
+
+struct TailArgs {
+  const lac_bf16x8 *Bp2, *Bp3;  // packed fragments: layer 2 (16 n-tiles), layer 3 (8 n-tiles)
+  const float *b2, *b3;
+  const __bf16 *A;              // h1 of the launch's rows, row-major, TH wide
+  float *Y;                     // layer 3 output (ldy)
+  long ldy;
+  int N3;                       // outputs of layer 3 (<= 256)
+  int nrt;                      // row tiles of TM
+};
+
+__global__ void __launch_bounds__(FT) k_mlp_tail(TailArgs a) {
+  // h2 tile [TM][TH] bf16; the 16-byte chunk c of row m is stored at
+  // c ^ (m & 15), so the A-fragment reads of 16 consecutive rows hit
+  // distinct banks
+  __shared__ __attribute__((aligned(16))) __bf16 sH[TM * TH];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int rt = (b & 7) * (nb >> 3) + (b >> 3);  // consecutive tiles on one XCD
+  if (rt >= a.nrt) return;                        // the whole block
+  const long rowb = (long)rt * TM;
+  const int wm = wave >> 2, wn = wave & 3;
+
+  // ---- layer 2: rows wm*64 + 32x, outputs wn*128 + 32y
+  lac_f32x16 acc[2][4];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[x][y][r] = 0.f;
+  // this lane's share of a stage: A row wave*16 + lane/4, chunk slot lane%4
+  // (holding chunk slot ^ ((row >> 1) & 3) of the row's 32 k); B fragments
+  // p = 4*wave + q (k-step p/16 of the stage, n-tile p%16)
+  const int arow = wave * 16 + (lane >> 2);
+  const __bf16 *asrc = a.A + (rowb + arow) * TH + ((lane & 3) ^ ((arow >> 1) & 3)) * 8;
+  auto issue = [&](int st, int s) {
+    __bf16 *base = sH + s * TBUF;
+    dma16(asrc + st * 32, base + wave * 16 * 32);
+    lac_bf16x8 *bb = reinterpret_cast<lac_bf16x8 *>(base + TSA);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int p = 4 * wave + q;
+      dma16(a.Bp2 + ((long)(2 * st + (p >> 4)) * (TH / 32) + (p & 15)) * 64 + lane, bb + p * 64);
+    }
+  };
+  auto compute = [&](int s) {
+    const __bf16 *cA = sH + s * TBUF;
+    const lac_bf16x8 *cB = reinterpret_cast<const lac_bf16x8 *>(cA + TSA);
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+      lac_bf16x8 av[2], bv[4];
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        const int m = wm * 64 + 32 * x + (lane & 31);
+        const int c = (2 * t + (lane >> 5)) ^ ((m >> 1) & 3);
+        av[x] = *reinterpret_cast<const lac_bf16x8 *>(&cA[m * 32 + c * 8]);
+      }
+#pragma unroll
+      for (int y = 0; y < 4; y++) bv[y] = cB[(t * 16 + wn * 4 + y) * 64 + lane];
+#pragma unroll
+      for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++)
+          acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[x], bv[y], acc[x][y], 0, 0, 0);
+    }
+  };
+  constexpr int NST = TKS / 2;  // stages
+  issue(0, 0);
+  issue(1, 1);
+#pragma unroll
+  for (int st = 0; st < NST; st++) {
+    // stage st has landed when at most stage st+1's five copies are pending
+    if (st + 1 < NST)
+      wait_vm<5>();
+    else
+      wait_vm<0>();
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+    compute(st % 3);
+    // into the buffer of stage st-1, which every wave finished before the barrier
+    if (st + 2 < NST) issue(st + 2, (st + 2) % 3);
+  }
+
+  // ---- layer 3 weights: n-tiles wn*2 + y, the first four k-steps now
+  const int nt0 = wn * 2;
+  const bool live0 = nt0 * 32 < a.N3, live1 = (nt0 + 1) * 32 < a.N3;  // uniform per wave
+  lac_bf16x8 bq[4][2];
+  auto bload = [&](int t, int slot) {
+    if (live0) bq[slot][0] = a.Bp3[((long)t * 8 + nt0) * 64 + lane];
+    if (live1) bq[slot][1] = a.Bp3[((long)t * 8 + nt0 + 1) * 64 + lane];
+  };
+#pragma unroll
+  for (int t = 0; t < 4; t++) bload(t, t);
+
+  // ---- h2 = relu(acc + b2) as bf16 into the tile, once every wave is done
+  // with the stage buffers (its reads of them have fed its MFMAs; a bare
+  // barrier, so the weight loads above stay in flight)
+  __asm__ volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  __asm__ volatile("" ::: "memory");
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const int n = wn * 128 + 32 * y + (lane & 31);
+      const float bn = a.b2[n];
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int m = wm * 64 + 32 * x + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const float v = acc[x][y][r] + bn;
+        sH[m * TH + (((n >> 3) ^ (m & 15)) << 3) + (n & 7)] = (__bf16)(v > 0.f ? v : 0.f);
+      }
+    }
+  __syncthreads();
+
+  // ---- layer 3: rows wm*64 + 32x, n-tiles wn*2 + y
+  lac_f32x16 acc3[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc3[x][y][r] = 0.f;
+#pragma unroll
+  for (int t = 0; t < TKS; t++) {
+    lac_bf16x8 cur0 = bq[t & 3][0], cur1 = bq[t & 3][1];
+    if (t + 4 < TKS) bload(t + 4, t & 3);
+    lac_bf16x8 av[2];
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      const int m = wm * 64 + 32 * x + (lane & 31);
+      const int c = (2 * t + (lane >> 5)) ^ (m & 15);
+      av[x] = *reinterpret_cast<const lac_bf16x8 *>(&sH[m * TH + c * 8]);
+    }
+#pragma unroll
+    for (int x = 0; x < 2; x++) {
+      if (live0) acc3[x][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[x], cur0, acc3[x][0], 0, 0, 0);
+      if (live1) acc3[x][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[x], cur1, acc3[x][1], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) {
+      const int n = (nt0 + y) * 32 + (lane & 31);
+      if (n >= a.N3) continue;
+      const float bn = a.b3[n];
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int m = wm * 64 + 32 * x + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        a.Y[(rowb + m) * a.ldy + n] = acc3[x][y][r] + bn;
+      }
+    }
+}
+
 // Training targets at columns cols[r]: ordinary-substep result (the state)
 // minus the long-substep result (fu, fv, fw, fpai).
 template <typename T>
@@ -3136,6 +3310,22 @@ struct Core : CoreBase {
     a.H = p->H1;
     a.ldh = p->nhid;
     hipLaunchKernelGGL(k_mlp_fused, grid(a.nN), dim3(FT), 0, stream, L.rows, a);
+    if (p->nhid == TH && p->nout <= 256 && p->wp_nt[1] == TH / 32 && p->wp_ks[1] == TKS && p->wp_nt[2] == 8 &&
+        p->wp_ks[2] == TKS && n % TM == 0) {
+      // layers 2 and 3 in one kernel
+      TailArgs t{};
+      t.Bp2 = reinterpret_cast<const lac_bf16x8 *>(p->Wp[1]);
+      t.Bp3 = reinterpret_cast<const lac_bf16x8 *>(p->Wp[2]);
+      t.b2 = p->bvec[1];
+      t.b3 = p->bvec[2];
+      t.A = p->H1;
+      t.Y = L.Y + r0 * L.nout;
+      t.ldy = L.nout;
+      t.N3 = p->nout;
+      t.nrt = n / TM;
+      hipLaunchKernelGGL(k_mlp_tail, dim3((unsigned)round_up((long)t.nrt, 8)), dim3(FT), 0, stream, t);
+      return;
+    }
     // layer 2
     a.Bp = reinterpret_cast<const lac_bf16x8 *>(p->Wp[1]);
     a.bnt = p->wp_nt[1];

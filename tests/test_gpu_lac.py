@@ -178,19 +178,21 @@ def test_learned_increment_beats_long_substep(precision):
         m.close()
 
 
-@pytest.mark.parametrize("shape", [(64, 64, 16), (70, 45, 20)])
-def test_fused_engine_matches_generic(shape):
+@pytest.mark.parametrize("shape,nhid", [((64, 64, 16), 128), ((70, 45, 20), 128), ((64, 64, 16), 512),
+                                        ((64, 48, 60), 512)])
+def test_fused_engine_matches_generic(shape, nhid):
     """The fused kernels (inputs gathered from the fields inside the MFMA
-    kernel, packed weights) against materialised features + the generic GEMM:
-    same trained network, one learned step each."""
+    kernel, packed weights; at 512 hidden units layers 2 and 3 in one kernel,
+    with 64 and with 240 outputs) against materialised features + the generic
+    GEMM: same trained network, one learned step each."""
     jx, iy, kz = shape
     case = small_config2(jx, iy, kz)
     runs = {}
     for engine in (0, 1):
         m, _, _ = _model(case, precision=dm.MLC_FP32)
-        corr = Corrector(m, nhid=128, samples_per_phase=1024, capacity=8192, chunk_rows=1024, check_every=0,
+        corr = Corrector(m, nhid=nhid, samples_per_phase=1024, capacity=8192, chunk_rows=1024, check_every=0,
                          seed=5, engine=engine)
-        print(f"case {jx}x{iy}x{kz} fp32 engine={engine}", flush=True)
+        print(f"case {jx}x{iy}x{kz} nhid={nhid} fp32 engine={engine}", flush=True)
         corr.collect(2)
         corr.train(200, 256, 1e-3)
         w = [corr.network(kz).get_weights(l) for l in range(3)]
