@@ -10,7 +10,8 @@ one FP32 field allocation (the corrector's phase-start snapshots of u and v,
 pitch x (iy + 4) x kz x 4 bytes each): the factor is the copied bytes over
 the counter's reading.  Only dispatches of the FP32 run are used (those after
 its device case build), grouped into the kernel kinds that bench.py times;
-one ``lac_mlp`` launch is the three layer kernels of one inference batch.
+one ``lac_mlp`` launch is the layer kernels of one inference batch (layer 1,
+then layers 2 and 3 in one kernel or in two).
 bench.py reads the result as ``roofline.traffic``.
 """
 from __future__ import annotations
@@ -30,7 +31,8 @@ KINDS = [
     ("k_sound_bs<", "sound_b"), ("k_sound_b<", "sound_b"), ("k_sound_c<", "sound_c"),
     ("k_adv_in<", "adv_in"), ("k_wafv<", "waf_v"), ("k_wafh<", "waf_h"), ("k_adv_out<", "adv_out"),
     ("k_epilogue<", "epilogue"), ("k_lac_features<", "lac_gather"), ("k_lac_apply<", "lac_apply"),
-    ("k_mlp_fused", "mlp_fused"), ("k_mlp_dense", "mlp_dense"), ("__amd_rocclr_copyBuffer", "copy"),
+    ("k_mlp_fused", "mlp_fused"), ("k_mlp_tail", "mlp_rest"), ("k_mlp_dense", "mlp_rest"),
+    ("__amd_rocclr_copyBuffer", "copy"),
 ]
 FP32_START = "k_case_build<float>"
 
@@ -97,13 +99,13 @@ def main(argv):
     wm, wc, wf, nw = per_kind(read_counter(dir_w, "WRITE_SIZE"), field_bytes)
     kinds = {}
     for k in fm:
-        if k in ("mlp_fused", "mlp_dense") or k not in wm:
+        if k in ("mlp_fused", "mlp_rest") or k not in wm:
             continue
         kinds[k] = {"fetch_bytes": fm[k], "write_bytes": wm[k], "bytes": fm[k] + wm[k], "dispatches": fc[k]}
-    if "mlp_fused" in fm and "mlp_dense" in fm:
-        per = fc["mlp_dense"] / fc["mlp_fused"]  # dense launches per batch
-        fb = fm["mlp_fused"] + per * fm["mlp_dense"]
-        wb = wm["mlp_fused"] + per * wm["mlp_dense"]
+    if "mlp_fused" in fm and "mlp_rest" in fm:
+        per = fc["mlp_rest"] / fc["mlp_fused"]  # launches after layer 1 per batch
+        fb = fm["mlp_fused"] + per * fm["mlp_rest"]
+        wb = wm["mlp_fused"] + per * wm["mlp_rest"]
         kinds["lac_mlp"] = {"fetch_bytes": fb, "write_bytes": wb, "bytes": fb + wb, "dispatches": fc["mlp_fused"]}
     out = {
         "workload": f"configuration 4, {jx}x{iy}x{kz}, FP32 state (FP64 Exner): kernels of the FP32 and "
@@ -111,7 +113,7 @@ def main(argv):
         "commands": [f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d prof -o run -- python3 bench.py {args}",
                      f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d prof -o run -- python3 bench.py {args}",
                      f"python3 -m dynamical_model_amd.traffic <fetch dir> <write dir> {jx} {iy} {kz} '{args}'"],
-        "units": "bytes per launch as bench.py times launches (lac_mlp: the three layer kernels of one batch)",
+        "units": "bytes per launch as bench.py times launches (lac_mlp: the layer kernels of one batch)",
         "calibration": {"copy_bytes": field_bytes, "fetch_factor": ff, "write_factor": wf,
                         "copies": min(nf, nw)},
         "kinds": kinds,

@@ -49,3 +49,19 @@ def test_traffic_reduction(tmp_path):
     assert abs(sb["write_bytes"] - 200.0 * 1024) < 1e-6
     mlp = d["kinds"]["lac_mlp"]  # one fused + two dense launches per batch
     assert abs(mlp["write_bytes"] - (10.0 + 2 * 3.0) * 1024) < 1e-6
+
+
+def test_traffic_reduction_tail_kernel(tmp_path):
+    """Layers 2 and 3 in one launch: one lac_mlp launch is layer 1 plus that kernel."""
+    pitch = (31 + 64 + 3 + 31) // 32 * 32
+    kib = pitch * (64 + 4) * 8 * 4 / 1024.0
+    fp32 = [("void mlc::k_case_build<float>(x)", 1.0), ("mlc::k_mlp_fused(x)", 10.0), ("mlc::k_mlp_tail(x)", 4.0),
+            ("mlc::k_mlp_fused(x)", 10.0), ("mlc::k_mlp_tail(x)", 6.0)]
+    _write(str(tmp_path / "f"), "FETCH_SIZE", fp32 + [("__amd_rocclr_copyBuffer", kib / 2)] * 2)
+    _write(str(tmp_path / "w"), "WRITE_SIZE", fp32 + [("__amd_rocclr_copyBuffer", kib)] * 2)
+    out = subprocess.run([sys.executable, "-m", "dynamical_model_amd.traffic", str(tmp_path / "f"),
+                          str(tmp_path / "w"), "64", "64", "8", "--x"], cwd=ROOT, capture_output=True,
+                         text=True, check=True).stdout
+    mlp = json.loads(out)["kinds"]["lac_mlp"]
+    assert mlp["dispatches"] == 2
+    assert abs(mlp["write_bytes"] - (10.0 + 5.0) * 1024) < 1e-6
