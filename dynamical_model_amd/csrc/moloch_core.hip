@@ -80,6 +80,26 @@ struct Geo {
   }
 };
 
+// Element i of an array: with an unsigned index the byte offset is formed in
+// 32 bits, so the load or store can take the SGPR-base plus 32-bit-offset
+// form (one VGPR per address); the caller keeps every array below 4 GiB.
+template <typename X>
+__device__ __forceinline__ X ldx(const X *b, unsigned i) {
+  return *reinterpret_cast<const X *>(reinterpret_cast<const char *>(b) + i * (unsigned)sizeof(X));
+}
+template <typename X>
+__device__ __forceinline__ X ldx(const X *b, long i) {
+  return b[i];
+}
+template <typename X>
+__device__ __forceinline__ void stx(X *b, unsigned i, X v) {
+  *reinterpret_cast<X *>(reinterpret_cast<char *>(b) + i * (unsigned)sizeof(X)) = v;
+}
+template <typename X>
+__device__ __forceinline__ void stx(X *b, long i, X v) {
+  b[i] = v;
+}
+
 // Per-run scalar parameters handed to every kernel.
 struct Par {
   double dx, dzita, anu2;
@@ -1071,7 +1091,8 @@ struct VField {
     F1 = R(0);
     F2 = R(0);
   }
-  __device__ __forceinline__ void step(const VCoef<R> &c, int m, int nk, R pm, R *wz, long c0, long pl) {
+  template <typename I>
+  __device__ __forceinline__ void step(const VCoef<R> &c, int m, int nk, R pm, R *wz, I c0, I pl) {
     const int kb = m - 2, kc = m - 4;
 #pragma unroll
     for (int q = 0; q < 3; q++) Pw[q] = Pw[q + 1];
@@ -1103,7 +1124,7 @@ struct VField {
       if (m - 3 >= 2 && m - 3 <= nk) f2 = waf_vflux(c.S[1], c.dtrdz, Ww[0], Ww[1], Ww[2], Ww[3]);
       R q = Ww[1];
       R zdv = (c.S[0] * c.ZU[0] - c.S[1] * c.ZD[0]) * q;
-      wz[c0 + (long)(kc - 1) * pl] = q - F2 * c.ZU[0] + f2 * c.ZD[0] + zdv;
+      stx(wz, (I)(c0 + (I)(kc - 1) * pl), q - F2 * c.ZU[0] + f2 * c.ZD[0] + zdv);
       F2 = f2;
     }
   }
@@ -1114,12 +1135,13 @@ struct VField {
 // zrfmu/zrfmd divisions are shared by the fields.  (An FP64 rider of the
 // horizontal kernel gets a launch of its own here: in one kernel its FP64
 // window and coefficients cost half the occupancy.)
-template <typename T, typename Q, int NF>
+template <typename T, typename Q, int NF, typename I>
 __global__ void __launch_bounds__(256) k_wafv(Dev<T> D, Par P, FieldSet<Q> F, double dta) {
   COL_IDX
   const int nk = g.nk;
-  const long c0 = g.at(j, i, 1);
-  const long pl = g.plane;
+  // element indices in I (unsigned: 32-bit byte offsets, as in k_wafh)
+  const I c0 = (I)g.at(j, i, 1);
+  const I pl = (I)g.plane;
   VCoef<Q> cq;
   VField<Q> vf[NF];
   // the loads of iteration m (s at interface m, fmz and the fields at level
@@ -1129,15 +1151,15 @@ __global__ void __launch_bounds__(256) k_wafv(Dev<T> D, Par P, FieldSet<Q> F, do
   Q n_p[NF];
   auto fetch = [&](int m) {
     const int mm = m <= nk ? m : nk, ms = m <= nk + 1 ? m : nk + 1;
-    const long xm = c0 + (long)(mm - 1) * pl;
-    n_s = D.s[c0 + (long)(ms - 1) * pl];
-    n_fz = D.fmz[xm];
-    n_ff = D.fmzf[xm + pl];
+    const I xm = (I)(c0 + (I)(mm - 1) * pl);
+    n_s = ldx(D.s, (I)(c0 + (I)(ms - 1) * pl));
+    n_fz = ldx(D.fmz, xm);
+    n_ff = ldx(D.fmzf, (I)(xm + pl));
 #pragma unroll
-    for (int f = 0; f < NF; f++) n_p[f] = F.in[f][xm];
+    for (int f = 0; f < NF; f++) n_p[f] = ldx(F.in[f], xm);
   };
   fetch(1);
-  cq.init((Q)D.fmzf[c0], dta, P.dzita);
+  cq.init((Q)ldx(D.fmzf, c0), dta, P.dzita);
   for (int m = 1; m <= nk + 4; m++) {
     const T sm = m <= nk + 1 ? n_s : T(0);
     const T fzm = n_fz, ff1 = n_ff;
@@ -1312,7 +1334,7 @@ __device__ __forceinline__ R waf_hrow(R (&w)[5], R &zs, R pp, const HCoef<R> &c,
 // Both horizontal sweeps of NF fields of type Q and ND (0 or 1) FP64 fields
 // for one strip of one level per wave; the field-independent coefficients
 // are evaluated once per row and precision.
-template <typename T, typename Q, int NF, int ND, bool ROT>
+template <typename T, typename Q, int NF, int ND, bool ROT, typename I>
 __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, FieldD FD, double dta, int ntj,
                                               int nti) {
   static_assert(ND == 0 || ND == 1, "at most one FP64 rider");
@@ -1334,8 +1356,11 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, Fi
   const int jp = jw == nj ? 1 : jw + 1, jm = jw == 1 ? nj : jw - 1;
   const Q dtrdx = (Q)(dta / P.dx), dtrdy = (Q)(dta / P.dx);
   const double dtrdx_d = dta / P.dx, dtrdy_d = dta / P.dx;
-  const long pl = (long)(k - 1) * g.plane;
-  auto at3 = [&](int jj, int ii) { return pl + (long)(ii + 1) * g.pj + (long)(jj + JOFF); };
+  // element indices in I: unsigned (32-bit byte offsets from the SGPR
+  // bases, one VGPR per address) when every array is below 4 GiB, else long
+  const I pl = (I)((long)(k - 1) * g.plane), pj = (I)g.pj;
+  auto at3 = [&](int jj, int ii) { return (I)(pl + (I)(ii + 1) * pj + (I)(jj + JOFF)); };
+  auto at2 = [&](int jj, int ii) { return (I)((I)(ii + 1) * pj + (I)(jj + JOFF)); };
   auto nxt = [&](int ii) { return ii == ni ? 1 : ii + 1; };
   const int lm1 = lane > 0 ? lane - 1 : 0, lm2 = lane > 1 ? lane - 2 : 0, lp1 = lane < 63 ? lane + 1 : 63;
 
@@ -1346,8 +1371,8 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, Fi
   double wd[5], zsd = 0.0;
   {
     // Courant number and amplitude of the south face of the first row
-    const T vv = D.v[at3(jw, i_c)];
-    const T rm = ROT ? T(0) : D.rmv[g.at2(jw, i_c)];
+    const T vv = ldx(D.v, at3(jw, i_c));
+    const T rm = ROT ? T(0) : ldx(D.rmv, at2(jw, i_c));
     Q mz, ma;
     double mzd, mad;
     if constexpr (ROT) {
@@ -1363,17 +1388,17 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, Fi
     }
 #pragma unroll
     for (int f = 0; f < NF; f++) {
-      w[f][0] = F.wz[f][at3(jw, i_m2)];
-      w[f][1] = F.wz[f][at3(jw, i_m1)];
-      w[f][2] = F.wz[f][at3(jw, i_c)];
-      w[f][3] = F.wz[f][at3(jw, i_p1)];
+      w[f][0] = ldx(F.wz[f], at3(jw, i_m2));
+      w[f][1] = ldx(F.wz[f], at3(jw, i_m1));
+      w[f][2] = ldx(F.wz[f], at3(jw, i_c));
+      w[f][3] = ldx(F.wz[f], at3(jw, i_p1));
       zs[f] = waf_mflux(mz, ma, w[f][0], w[f][1], w[f][2], w[f][3]);
     }
     if (ND) {
-      wd[0] = FD.wz[at3(jw, i_m2)];
-      wd[1] = FD.wz[at3(jw, i_m1)];
-      wd[2] = FD.wz[at3(jw, i_c)];
-      wd[3] = FD.wz[at3(jw, i_p1)];
+      wd[0] = ldx(FD.wz, at3(jw, i_m2));
+      wd[1] = ldx(FD.wz, at3(jw, i_m1));
+      wd[2] = ldx(FD.wz, at3(jw, i_c));
+      wd[3] = ldx(FD.wz, at3(jw, i_p1));
       zsd = waf_mflux(mzd, mad, wd[0], wd[1], wd[2], wd[3]);
     }
   }
@@ -1382,39 +1407,39 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, Fi
   // the previous row), fmz east and west, u here and east, the map factors
   auto row_in = [&](int ic, int in) {
     HRow<T> o;
-    o.vn = D.v[at3(jw, in)];
-    o.fn = D.fmz[at3(jw, in)];
-    o.fe = D.fmz[at3(jp, ic)];
-    o.fw = D.fmz[at3(jm, ic)];
-    o.uc = D.u[at3(jw, ic)];
-    o.up = D.u[at3(jp, ic)];
-    o.rvn = D.rmv[g.at2(jw, in)];
+    o.vn = ldx(D.v, at3(jw, in));
+    o.fn = ldx(D.fmz, at3(jw, in));
+    o.fe = ldx(D.fmz, at3(jp, ic));
+    o.fw = ldx(D.fmz, at3(jm, ic));
+    o.uc = ldx(D.u, at3(jw, ic));
+    o.up = ldx(D.u, at3(jp, ic));
+    o.rvn = ldx(D.rmv, at2(jw, in));
     if constexpr (ROT) {
-      o.m0 = D.mx[g.at2(jw, ic)];
-      o.m1 = D.mu[g.at2(jw, ic)];
+      o.m0 = ldx(D.mx, at2(jw, ic));
+      o.m1 = ldx(D.mu, at2(jw, ic));
       o.m2 = T(0);
     } else {
-      o.m0 = D.mx2[g.at2(jw, ic)];
-      o.m1 = D.rmu[g.at2(jw, ic)];
-      o.m2 = D.rmu[g.at2(jp, ic)];
+      o.m0 = ldx(D.mx2, at2(jw, ic));
+      o.m1 = ldx(D.rmu, at2(jw, ic));
+      o.m2 = ldx(D.rmu, at2(jp, ic));
     }
     return o;
   };
   // carried south values of the current row
-  T v_c = D.v[at3(jw, i_c)], rv_c = D.rmv[g.at2(jw, i_c)];
-  T f_s = D.fmz[at3(jw, i_m1)], f_c = D.fmz[at3(jw, i_c)];
+  T v_c = ldx(D.v, at3(jw, i_c)), rv_c = ldx(D.rmv, at2(jw, i_c));
+  T f_s = ldx(D.fmz, at3(jw, i_m1)), f_c = ldx(D.fmz, at3(jw, i_c));
   HRow<T> nx = row_in(i_c, i_p1);
   // loads of the first row
   Q nw[NF], np[NF];
   double nwd = 0.0, npd = 0.0;
 #pragma unroll
   for (int f = 0; f < NF; f++) {
-    nw[f] = F.wz[f][at3(jw, i_p2)];
-    np[f] = F.in[f][at3(jw, i_c)];
+    nw[f] = ldx(F.wz[f], at3(jw, i_p2));
+    np[f] = ldx(F.in[f], at3(jw, i_c));
   }
   if (ND) {
-    nwd = FD.wz[at3(jw, i_p2)];
-    npd = FD.in[at3(jw, i_c)];
+    nwd = ldx(FD.wz, at3(jw, i_p2));
+    npd = ldx(FD.in, at3(jw, i_c));
   }
   for (int r = 0; r < nrows; r++) {
     Q pp[NF];
@@ -1430,26 +1455,26 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, Fi
       const int i_n = nxt(i_p2);
 #pragma unroll
       for (int f = 0; f < NF; f++) {
-        nw[f] = F.wz[f][at3(jw, i_n)];
-        np[f] = F.in[f][at3(jw, i_p1)];
+        nw[f] = ldx(F.wz[f], at3(jw, i_n));
+        np[f] = ldx(F.in[f], at3(jw, i_p1));
       }
       if (ND) {
-        nwd = FD.wz[at3(jw, i_n)];
-        npd = FD.in[at3(jw, i_p1)];
+        nwd = ldx(FD.wz, at3(jw, i_n));
+        npd = ldx(FD.in, at3(jw, i_p1));
       }
       nx = row_in(i_p1, i_p2);
     }
     const HCoef<Q> cq = waf_hcoef<Q, ROT>(ci, v_c, rv_c, f_s, f_c, dtrdx, dtrdy);
-    const long xo = at3(ja <= nj ? ja : jw, i_c);
+    const I xo = at3(ja <= nj ? ja : jw, i_c);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
       const Q o = waf_hrow<Q, ROT>(w[f], zs[f], pp[f], cq, lm1, lm2, lp1);
-      if (store) F.out[f][xo] = o;
+      if (store) stx(F.out[f], xo, o);
     }
     if (ND) {
       const HCoef<double> cd = waf_hcoef<double, ROT>(ci, v_c, rv_c, f_s, f_c, dtrdx_d, dtrdy_d);
       const double o = waf_hrow<double, ROT>(wd, zsd, ppd, cd, lm1, lm2, lp1);
-      if (store) FD.out[xo] = o;
+      if (store) stx(FD.out, xo, o);
     }
     // the north values become the next row's south ones
     v_c = ci.vn;
@@ -3021,6 +3046,10 @@ struct Core : CoreBase {
     launch(KK_WAF_Z, fbytes(1 - q, q), k_waf_z<T, Q>, pt_grid(), blk(), D, P, pp, (const Q *)p0, dta);
   }
 
+  // every 3-D array (kz + 1 levels, FP64 at most) below 4 GiB: kernels may
+  // address them by 32-bit byte offsets
+  bool small_offsets() const { return (double)g.plane * (g.nk + 1) * sizeof(double) < 4294967296.0; }
+
   // wafone on a group of fields of one storage type: k_wafv then k_wafh; each
   // field's result buffer then takes the field's place.
   // One group: NF fields of type Q (in *fld[f], scratch wzs[f], results into
@@ -3040,25 +3069,37 @@ struct Core : CoreBase {
     if (ND) FD = FieldD{*dfld, dwz, *dout};
     const double q = sizeof(Q) == sizeof(double) ? 1.0 : 0.0;
     // vertical: s, fmz, fmzf + the fields -> wz (the rider on its own)
-    launch(KK_WAF_V, fbytes(3, 0) + (q ? fbytes(0, NF) : fbytes(NF, 0)), k_wafv<T, Q, NF>, col_grid(), blk(), D, P,
-           F, dta);
+    const double vb = fbytes(3, 0) + (q ? fbytes(0, NF) : fbytes(NF, 0));
+    if (small_offsets())
+      launch(KK_WAF_V, vb, k_wafv<T, Q, NF, unsigned>, col_grid(), blk(), D, P, F, dta);
+    else
+      launch(KK_WAF_V, vb, k_wafv<T, Q, NF, long>, col_grid(), blk(), D, P, F, dta);
     if (ND) {
       FieldSet<double> Fd;
       for (int f = 0; f < MAXF; f++) Fd.in[f] = nullptr, Fd.wz[f] = nullptr, Fd.out[f] = nullptr;
       Fd.in[0] = FD.in;
       Fd.wz[0] = FD.wz;
-      launch(KK_WAF_V, fbytes(0, 1), k_wafv<T, double, 1>, col_grid(), blk(), D, P, Fd, dta);
+      if (small_offsets())
+        launch(KK_WAF_V, fbytes(0, 1), k_wafv<T, double, 1, unsigned>, col_grid(), blk(), D, P, Fd, dta);
+      else
+        launch(KK_WAF_V, fbytes(0, 1), k_wafv<T, double, 1, long>, col_grid(), blk(), D, P, Fd, dta);
     }
     // horizontal: u, v, fmz + the fields' wz and starting values -> results
     const int ntj = (g.nj + HJ - 1) / HJ, nti = (g.ni + HR - 1) / HR;
     const long total = ((long)ntj * nti * g.nk + 3) / 4;  // blocks of four strips
     const double hb = (q ? fbytes(0, NF) : fbytes(NF, 0)) + fbytes(0, ND);
-    if (prm.rotllr)
-      launch(KK_WAF_H, hb, k_wafh<T, Q, NF, ND, true>, dim3((unsigned)round_up(total, 8)), dim3(256), D, P, F, FD,
-             dta, ntj, nti);
-    else
-      launch(KK_WAF_H, hb, k_wafh<T, Q, NF, ND, false>, dim3((unsigned)round_up(total, 8)), dim3(256), D, P, F, FD,
-             dta, ntj, nti);
+    const dim3 hg((unsigned)round_up(total, 8));
+    if (small_offsets()) {
+      if (prm.rotllr)
+        launch(KK_WAF_H, hb, k_wafh<T, Q, NF, ND, true, unsigned>, hg, dim3(256), D, P, F, FD, dta, ntj, nti);
+      else
+        launch(KK_WAF_H, hb, k_wafh<T, Q, NF, ND, false, unsigned>, hg, dim3(256), D, P, F, FD, dta, ntj, nti);
+    } else {
+      if (prm.rotllr)
+        launch(KK_WAF_H, hb, k_wafh<T, Q, NF, ND, true, long>, hg, dim3(256), D, P, F, FD, dta, ntj, nti);
+      else
+        launch(KK_WAF_H, hb, k_wafh<T, Q, NF, ND, false, long>, hg, dim3(256), D, P, F, FD, dta, ntj, nti);
+    }
     for (int f = 0; f < NF; f++) std::swap(*fld[f], outs[f]);
     if (ND) std::swap(*dfld, *dout);
   }

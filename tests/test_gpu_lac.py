@@ -179,12 +179,13 @@ def test_learned_increment_beats_long_substep(precision):
 
 
 @pytest.mark.parametrize("shape,nhid", [((64, 64, 16), 128), ((70, 45, 20), 128), ((64, 64, 16), 512),
-                                        ((64, 48, 60), 512)])
+                                        ((64, 48, 60), 512), ((64, 64, 9), 512)])
 def test_fused_engine_matches_generic(shape, nhid):
     """The fused kernels (inputs gathered from the fields inside the MFMA
     kernel, packed weights; at 512 hidden units layers 2 and 3 in one kernel,
-    with 64 and with 240 outputs) against materialised features + the generic
-    GEMM: same trained network, one learned step each."""
+    with 64, 240 and 36 outputs, the last filling one 32-wide tile and part of
+    the next) against materialised features + the generic GEMM: same trained
+    network, one learned step each."""
     jx, iy, kz = shape
     case = small_config2(jx, iy, kz)
     runs = {}
