@@ -658,16 +658,17 @@ __global__ void __launch_bounds__(256) k_sound_b2(Dev<T> D, Par P) {
 // hold the substep-start winds and each interior value read gets divdamp's
 // increment (:865-872) here, from the horizontal divergence zdiv2 of phase
 // A, in the operand order of k_sound_b, so the winds agree bit for bit.
-template <typename T, bool DAMP>
+template <typename T, bool DAMP, typename I>
 __global__ void __launch_bounds__(256) k_sound_c(Dev<T> D, Par P, double dts) {
   COL_IDX
   const double dtrdx = dts / P.dx, dtrdy = dts / P.dx;
-  const long c2 = g.at2(j, i);
-  const double zcx = dtrdx * (double)D.mu[c2];
-  const double zcy = P.rotllr ? dtrdy : dtrdy * (double)D.mv[c2];
-  const double hx = (double)D.hx[c2], hy = (double)D.hy[c2];
-  const double coru = (double)D.coru[c2], corv = (double)D.corv[c2];
-  const long pj = g.pj;
+  // element indices in I (unsigned: 32-bit byte offsets from the SGPR bases)
+  const I c2 = (I)g.at2(j, i);
+  const double zcx = dtrdx * (double)ldx(D.mu, c2);
+  const double zcy = P.rotllr ? dtrdy : dtrdy * (double)ldx(D.mv, c2);
+  const double hx = (double)ldx(D.hx, c2), hy = (double)ldx(D.hy, c2);
+  const double coru = (double)ldx(D.coru, c2), corv = (double)ldx(D.corv, c2);
+  const I pj = (I)g.pj, plane = (I)g.plane;
   // divdamp of the winds read (DAMP): the points are u(j,i), u(j,i-1),
   // u(j+1,i), u(j+1,i-1) and v(j,i), v(j-1,i), v(j-1,i+1), v(j,i+1); each
   // gets xknu*ddamp*m*(zdiv2 difference) if it is an interior point.  Every
@@ -677,31 +678,31 @@ __global__ void __launch_bounds__(256) k_sound_c(Dev<T> D, Par P, double dts) {
   const bool jlo = j >= 2, jhi = j <= g.nj - 1, ilo = i >= 2, ihi = i <= g.ni - 1;
   T mu_c = 0, mu_im = 0, mu_jp = 0, mu_jpim = 0, mv_c = 0, mv_jm = 0, mv_jmip = 0, mv_ip = 0;
   if (DAMP) {
-    mu_c = D.mu[c2];
-    mu_im = D.mu[c2 - pj];
-    mu_jp = D.mu[c2 + 1];
-    mu_jpim = D.mu[c2 + 1 - pj];
-    mv_c = D.mv[c2];
-    mv_jm = D.mv[c2 - 1];
-    mv_jmip = D.mv[c2 - 1 + pj];
-    mv_ip = D.mv[c2 + pj];
+    mu_c = ldx(D.mu, c2);
+    mu_im = ldx(D.mu, (I)(c2 - pj));
+    mu_jp = ldx(D.mu, (I)(c2 + 1));
+    mu_jpim = ldx(D.mu, (I)(c2 + 1 - pj));
+    mv_c = ldx(D.mv, c2);
+    mv_jm = ldx(D.mv, (I)(c2 - 1));
+    mv_jmip = ldx(D.mv, (I)(c2 - 1 + pj));
+    mv_ip = ldx(D.mv, (I)(c2 + pj));
   }
-  long x = g.at(j, i, 1);
-  double dwc = (double)D.deltaw[x], dwjm = (double)D.deltaw[x - 1], dwim = (double)D.deltaw[x - pj];
-  for (int k = 1; k <= g.nk; k++, x += g.plane) {
-    long xp = x + g.plane;
-    double dwc1 = (double)D.deltaw[xp], dwjm1 = (double)D.deltaw[xp - 1], dwim1 = (double)D.deltaw[xp - pj];
-    double tc = (double)D.tetav[x];
-    double pc = D.pai[x];
+  I x = (I)g.at(j, i, 1);
+  double dwc = (double)ldx(D.deltaw, x), dwjm = (double)ldx(D.deltaw, (I)(x - 1)), dwim = (double)ldx(D.deltaw, (I)(x - pj));
+  for (int k = 1; k <= g.nk; k++, x += plane) {
+    I xp = x + plane;
+    double dwc1 = (double)ldx(D.deltaw, xp), dwjm1 = (double)ldx(D.deltaw, (I)(xp - 1)), dwim1 = (double)ldx(D.deltaw, (I)(xp - pj));
+    double tc = (double)ldx(D.tetav, x);
+    double pc = ldx(D.pai, x);
     double gzh = D.gzitakh[k - 1];
-    T u_c = D.u[x], u_im = D.u[x - pj], u_jp = D.u[x + 1], u_jpim = D.u[x + 1 - pj];
-    T v_c = D.v[x], v_jm = D.v[x - 1], v_jmip = D.v[x - 1 + pj], v_ip = D.v[x + pj];
+    T u_c = ldx(D.u, x), u_im = ldx(D.u, (I)(x - pj)), u_jp = ldx(D.u, (I)(x + 1)), u_jpim = ldx(D.u, (I)(x + 1 - pj));
+    T v_c = ldx(D.v, x), v_jm = ldx(D.v, (I)(x - 1)), v_jmip = ldx(D.v, (I)(x - 1 + pj)), v_ip = ldx(D.v, (I)(x + pj));
     const T u_start = u_c, v_start = v_c;
     if (DAMP) {
       const T xk = (T)D.xknu[k - 1];
-      const T z00 = D.zdh[x], zm0 = D.zdh[x - 1], zp0 = D.zdh[x + 1];
-      const T z0m = D.zdh[x - pj], zmm = D.zdh[x - 1 - pj], zpm = D.zdh[x + 1 - pj];
-      const T z0p = D.zdh[x + pj], zmp = D.zdh[x - 1 + pj];
+      const T z00 = ldx(D.zdh, x), zm0 = ldx(D.zdh, (I)(x - 1)), zp0 = ldx(D.zdh, (I)(x + 1));
+      const T z0m = ldx(D.zdh, (I)(x - pj)), zmm = ldx(D.zdh, (I)(x - 1 - pj)), zpm = ldx(D.zdh, (I)(x + 1 - pj));
+      const T z0p = ldx(D.zdh, (I)(x + pj)), zmp = ldx(D.zdh, (I)(x - 1 + pj));
       // the operand order of k_sound_b's level update
       const T du_c = u_c + xk * ddamp * mu_c * (z00 - zm0);
       const T du_im = u_im + xk * ddamp * mu_im * (z0m - zmm);
@@ -731,20 +732,20 @@ __global__ void __launch_bounds__(256) k_sound_c(Dev<T> D, Par P, double dts) {
     // Equation 17
     {
       double zfz = kEgrav * dts + 0.25 * (dwjm + dwjm1 + dwc + dwc1);
-      double zrom1u = 0.5 * kCpd * ((double)D.tetav[x - 1] + tc);
+      double zrom1u = 0.5 * kCpd * ((double)ldx(D.tetav, (I)(x - 1)) + tc);
       double zcor1u = coru * dts * 0.25 * ((double)v_c + (double)v_jm + (double)v_jmip + (double)v_ip);
-      double un = (double)u_c + zcor1u - zfz * hx * gzh - zcx * zrom1u * (pc - D.pai[x - 1]);
-      D.un[x] = (T)un;
-      if (DAMP && D.iu) D.iu[x] = (float)((T)un - u_start);
+      double un = (double)u_c + zcor1u - zfz * hx * gzh - zcx * zrom1u * (pc - ldx(D.pai, (I)(x - 1)));
+      stx(D.un, x, (T)un);
+      if (DAMP && D.iu) stx(D.iu, x, (float)((T)un - u_start));
     }
     // Equation 18
     {
       double zfz = kEgrav * dts + 0.25 * (dwim + dwim1 + dwc + dwc1);
-      double zrom1v = 0.5 * kCpd * ((double)D.tetav[x - pj] + tc);
+      double zrom1v = 0.5 * kCpd * ((double)ldx(D.tetav, (I)(x - pj)) + tc);
       double zcor1v = corv * dts * 0.25 * ((double)u_c + (double)u_im + (double)u_jp + (double)u_jpim);
-      double vn = (double)v_c - zcor1v - zfz * hy * gzh - zcy * zrom1v * (pc - D.pai[x - pj]);
-      D.vn[x] = (T)vn;
-      if (DAMP && D.iv) D.iv[x] = (float)((T)vn - v_start);
+      double vn = (double)v_c - zcor1v - zfz * hy * gzh - zcy * zrom1v * (pc - ldx(D.pai, (I)(x - pj)));
+      stx(D.vn, x, (T)vn);
+      if (DAMP && D.iv) stx(D.iv, x, (float)((T)vn - v_start));
     }
     dwc = dwc1;
     dwjm = dwjm1;
@@ -3016,10 +3017,17 @@ struct Core : CoreBase {
              (const double *)D.xknu, prm.anu2);
       launch(KK_SOUND_B2, fbytes(1, 2), k_sound_b2<T>, col_grid(), blk(), D, P);
     }
-    if (seg)
-      launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T, true>, col_grid(), blk(), D, P, dts);
+    // 32-bit offsets for the FP32 state only: the FP64 kernel measured
+    // slower with them (79 against 63 ms per 2048 x 2048 x 60 step)
+    const bool so = sizeof(T) == sizeof(float) && small_offsets();
+    if (seg && so)
+      launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T, true, unsigned>, col_grid(), blk(), D, P, dts);
+    else if (seg)
+      launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T, true, long>, col_grid(), blk(), D, P, dts);
+    else if (so)
+      launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T, false, unsigned>, col_grid(), blk(), D, P, dts);
     else
-      launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T, false>, col_grid(), blk(), D, P, dts);
+      launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T, false, long>, col_grid(), blk(), D, P, dts);
     std::swap(D.u, D.un);
     std::swap(D.v, D.vn);
   }
