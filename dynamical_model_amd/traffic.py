@@ -9,7 +9,12 @@ per dispatch.  Both are calibrated on the run's own device-to-device copies of
 one FP32 field allocation (the corrector's phase-start snapshots of u and v,
 pitch x (iy + 4) x kz x 4 bytes each): the factor is the copied bytes over
 the counter's reading.  Only dispatches of the FP32 run are used (those after
-its device case build), grouped into the kernel kinds that bench.py times;
+its device case build); the kinds are counted over its last step (from its
+last prologue on), so that with two or more steps they are the steady state
+of the learned loop (the first learned step checks its corrector, which
+takes the phase-start copies the calibration uses and reads them back in
+place of the increments the substep kernels write otherwise), grouped into
+the kernel kinds that bench.py times;
 one ``lac_mlp`` launch is the layer kernels of one inference batch (layer 1,
 then layers 2 and 3 in one kernel or in two).
 bench.py reads the result as ``roofline.traffic``.
@@ -35,6 +40,7 @@ KINDS = [
     ("__amd_rocclr_copyBuffer", "copy"),
 ]
 FP32_START = "k_case_build<float>"
+STEP_START = "k_prologue<float>"
 
 
 def kind_of(name: str) -> str | None:
@@ -69,13 +75,14 @@ def per_kind(rows, field_bytes: float):
     start = next((n for n, (_, name, _, _) in enumerate(rows) if FP32_START in name), None)
     if start is None:
         raise SystemExit(f"no {FP32_START} dispatch: not a bench run with an FP32 variant")
+    last = max((n for n, (_, name, _, _) in enumerate(rows) if n >= start and STEP_START in name), default=start)
     acc = defaultdict(list)
     copies = defaultdict(list)
-    for _, name, grid, val in rows[start:]:
+    for n, (_, name, grid, val) in enumerate(rows[start:], start):
         k = kind_of(name)
         if k == "copy":
             copies[grid].append(val)
-        elif k:
+        elif k and n >= last:
             acc[k].append(val)
     # the snapshot copies (one launch size for all) move u, v (one FP32 field
     # each), w (kz + 1 levels) and pi (FP64): u and v are the smallest

@@ -51,6 +51,27 @@ def test_traffic_reduction(tmp_path):
     assert abs(mlp["write_bytes"] - (10.0 + 2 * 3.0) * 1024) < 1e-6
 
 
+def test_traffic_counts_the_last_step(tmp_path):
+    """Kinds are counted from the FP32 run's last prologue on; the copies of
+    the whole FP32 run calibrate."""
+    pitch = (31 + 64 + 3 + 31) // 32 * 32
+    kib = pitch * (64 + 4) * 8 * 4 / 1024.0
+    fp32 = [("void mlc::k_case_build<float>(x)", 1.0), ("void mlc::k_prologue<float>(x)", 5.0),
+            ("void mlc::k_lac_features<float>(x)", 100.0)] + [("__amd_rocclr_copyBuffer", kib / 2)] * 2 + [
+        ("void mlc::k_prologue<float>(x)", 7.0), ("void mlc::k_lac_features<float>(x)", 40.0)]
+    _write(str(tmp_path / "f"), "FETCH_SIZE", fp32)
+    _write(str(tmp_path / "w"), "WRITE_SIZE", [(n, v * 2 if "copyBuffer" in n else v) for n, v in fp32])
+    out = subprocess.run([sys.executable, "-m", "dynamical_model_amd.traffic", str(tmp_path / "f"),
+                          str(tmp_path / "w"), "64", "64", "8", "--x"], cwd=ROOT, capture_output=True,
+                         text=True, check=True).stdout
+    d = json.loads(out)
+    assert abs(d["calibration"]["fetch_factor"] - 2.0) < 1e-12
+    g = d["kinds"]["lac_gather"]
+    assert g["dispatches"] == 1
+    assert abs(g["fetch_bytes"] - 40.0 * 1024 * 2.0) < 1e-6
+    assert abs(d["kinds"]["prologue"]["fetch_bytes"] - 7.0 * 1024 * 2.0) < 1e-6
+
+
 def test_traffic_reduction_tail_kernel(tmp_path):
     """Layers 2 and 3 in one launch: one lac_mlp launch is layer 1 plus that kernel."""
     pitch = (31 + 64 + 3 + 31) // 32 * 32
