@@ -6,7 +6,8 @@ or halo mistake at this size would break:
 
 * the fused kernels (grouped advection, segmented column solve) give the
   same bits as the one-kernel-per-phase path that follows the reference's
-  loop nests (and is itself held to the oracle on small grids);
+  loop nests (and is itself held to the oracle on small grids), also at
+  2976 x 2976 x 60, where arrays pass 4 GiB;
 * the step commutes with a periodic shift of the whole state and geometry
   (MPI halo order and no Coriolis force, the conditions under which the
   reference's step is translation-equivariant): shifting the input by
@@ -50,6 +51,26 @@ def test_fullsize_fused_equals_unfused():
             fa, fb = dm.interior(a.get_field(n)), dm.interior(b.get_field(n))
             assert np.isfinite(fa).all(), n
             np.testing.assert_array_equal(fa, fb, err_msg=n)
+    finally:
+        a.close()
+        b.close()
+
+
+def test_large_arrays_fused_equals_unfused():
+    """At 2976 x 2976 x 60 every FP64 array (kz + 1 levels) exceeds 4 GiB,
+    so the kernels that otherwise address by 32-bit byte offsets (grouped
+    WAF, DAMP momentum) run their 64-bit form: one FP32 step, bit-identical
+    to the one-kernel-per-phase path."""
+    case = _case(jx=2976, iy=2976)
+    print(f"{case.jx}x{case.iy}x{case.kz} FP32: fused vs unfused, one step (64-bit offsets)", flush=True)
+    a = _step_device_case(case, dm.case_params(case, precision=dm.MLC_FP32, unfused=0))
+    b = _step_device_case(case, dm.case_params(case, precision=dm.MLC_FP32, unfused=1))
+    try:
+        for n in PROGNOSTIC:
+            fa, fb = dm.interior(a.get_field(n)), dm.interior(b.get_field(n))
+            assert np.isfinite(fa).all(), n
+            np.testing.assert_array_equal(fa, fb, err_msg=n)
+            del fa, fb
     finally:
         a.close()
         b.close()
