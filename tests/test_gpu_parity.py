@@ -118,6 +118,42 @@ def test_fp64_ten_steps():
     _check_fp64(gpu, ora, 10, "fp64 bubble 10 steps")
 
 
+def _add_condensates(hs, c):
+    """A periodic cloud of all four condensate species in the host state
+    (halos included), so that the fulleq moist coefficients (:697-706), the
+    epilogue's condensate loading and the advection of qc, qi, qr, qs act on
+    nonzero values."""
+    kz, ny, nx = hs.fields["qc"].shape
+    jj, ii = np.meshgrid((np.arange(nx) - 2) % c.jx, (np.arange(ny) - 2) % c.iy)
+    bump = 0.25 * (1 + np.cos(2 * np.pi * jj / c.jx)) * (1 + np.cos(2 * np.pi * ii / c.iy))
+    prof = np.exp(-((np.arange(kz) - 0.6 * kz) / (0.2 * kz)) ** 2)[:, None, None]
+    for n, amp in (("qc", 3e-4), ("qr", 1e-4), ("qi", 5e-5), ("qs", 5e-5)):
+        hs.fields[n][:] = amp * prof * bump[None]
+
+
+def test_fp64_saturated_cloudy_bubble():
+    """Near-saturated air (RH 99 %) around a warm bubble, with clouds: the
+    column solve's moist term (qv > 0.96 qsat where w > 0.1, :645-657) and
+    the fulleq coefficients act, which the other cases (RH 50-60 %, no
+    condensates) never reach.  The regime is checked on the oracle's state."""
+    c = dm.case_config(1, jx=48, iy=40, kz=20, hill_hw=8.0, bubble_rcells=6.0, rh=0.99)
+    hs = dm.build_case(c)
+    _add_condensates(hs, c)
+    params = dm.case_params(c)
+    print(f"saturated cloudy bubble {c.jx}x{c.iy}x{c.kz}, 3 steps", flush=True)
+    gpu = dm.run_case(hs, params, 3)
+    ora = run_oracle(hs, params, 3)
+    qv, qsat, w = dm.interior(ora.field("qv")), dm.interior(ora.field("qsat")), dm.interior(ora.field("w"))
+    assert np.count_nonzero((qv > 0.96 * qsat) & (w[: qv.shape[0]] > 0.1)) > 0
+    assert np.abs(dm.interior(ora.field("qc"))).max() > 1e-4
+    _check_fp64(gpu, ora, 3, "fp64 saturated cloudy bubble", ALL + ["qc", "qi", "qr", "qs"])
+    # the same case in FP32: default kernels bit-identical to the reference-order ones
+    a = dm.run_case(hs, dm.case_params(c, precision=dm.MLC_FP32, unfused=0), 3)
+    b = dm.run_case(hs, dm.case_params(c, precision=dm.MLC_FP32, unfused=1), 3)
+    for n in PROGNOSTIC + ["qc", "qi", "qr", "qs"]:
+        np.testing.assert_array_equal(dm.interior(a[n]), dm.interior(b[n]), err_msg=n)
+
+
 def test_diagnostics_match_oracle():
     c = dm.case_config(0, jx=40, iy=36, kz=16)
     hs = dm.build_case(c)
