@@ -2377,11 +2377,13 @@ __global__ void __launch_bounds__(256) k_diag_final(double *partial, long nblk, 
       }
     __syncthreads();
   }
-  if (tid == 0)
+  if (tid == 0) {
     for (int f = 0; f < 3; f++) {
       out[2 * f] = scn[f][0] > 0 ? smx[f][0] : NAN;
       out[2 * f + 1] = scn[f][0] > 0 ? smn[f][0] : NAN;
     }
+    out[6] = scn[0][0];  // ps values that are not NaN
+  }
 }
 
 // ============================================================ case builder
@@ -3754,7 +3756,7 @@ struct Core : CoreBase {
     if (lprint) {
       launch(KK_DIAG, fbytes(2, 0), k_diag_partial<T>, col_grid(), blk(), D);
       launch(KK_DIAG, 0.0, k_diag_final, dim3(1), dim3(256), D.diag, nblk_diag, d_diag_out);
-      double out[6];
+      double out[7];
       HIP_TRY(hipMemcpyAsync(out, d_diag_out, sizeof(out), hipMemcpyDeviceToHost, stream));
       HIP_TRY(hipStreamSynchronize(stream));
       diag->ps_max = out[0];
@@ -3763,7 +3765,9 @@ struct Core : CoreBase {
       diag->u_min = out[3];
       diag->v_max = out[4];
       diag->v_min = out[5];
-      diag->cfl_violation = std::isnan(out[0]) || std::isnan(out[1]) || std::isinf(out[0]) || std::isinf(out[1]);
+      // any NaN in ps, or NaN/Inf extrema (the extrema skip NaN values)
+      diag->cfl_violation = out[6] < (double)g.nj * g.ni || std::isnan(out[0]) || std::isnan(out[1]) ||
+                            std::isinf(out[0]) || std::isinf(out[1]);
     }
     return MLC_OK;
   }

@@ -125,7 +125,10 @@ typedef struct mlc_params {
 /* lprint diagnostics of nompi/mod_moloch.F90:395-419 */
 typedef struct mlc_diag {
   double ps_max, ps_min, u_max, u_min, v_max, v_min;
-  int cfl_violation;  /* ps max/min is NaN or Inf                          */
+  int cfl_violation;  /* ps holds a NaN, or its max/min is Inf: the       */
+                      /* reference tests its maxval/minval (:402-403),    */
+                      /* whose NaN handling is compiler-dependent; the    */
+                      /* extrema here skip NaN values                     */
 } mlc_diag;
 
 typedef struct mlc_handle mlc_handle;

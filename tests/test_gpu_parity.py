@@ -171,6 +171,34 @@ def test_diagnostics_match_oracle():
     assert d.cfl_violation == 0 == o.S.cfl_violation
 
 
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+def test_cfl_violation_detected(bad):
+    """A NaN or an Inf in the Exner function at one surface point: after one
+    step ps is NaN in a patch of about a sixth of this grid, its extrema over
+    the rest are finite, and the lprint check of both the GPU and the oracle
+    flags the step (:402-411) with the same extrema; as in the nompi tree,
+    the run goes on (one more step)."""
+    c = dm.case_config(0, jx=160, iy=120, kz=12)
+    hs = dm.build_case(c)
+    hs.fields["pai"][c.kz - 1, 2 + 60, 2 + 80] = bad
+    params = dm.case_params(c)
+    m = dm.Moloch()
+    m.allocate_moloch(c.jx, c.iy, c.kz, params)
+    m.load_state(hs)
+    m.init_moloch()
+    d1 = m.moloch(1, True)
+    d2 = m.moloch(2, True)
+    m.close()
+    o = run_oracle(hs, params, 0)
+    o.moloch(1, True)
+    print(f"{bad}: gpu {d1.cfl_violation} {d2.cfl_violation} oracle {o.S.cfl_violation}", flush=True)
+    assert d1.cfl_violation == 1 == o.S.cfl_violation
+    assert d2.cfl_violation == 1
+    got = [d1.ps_max, d1.ps_min, d1.u_max, d1.u_min, d1.v_max, d1.v_min]
+    assert np.isfinite(got[:2]).all()
+    np.testing.assert_allclose(got, list(o.S.diag), rtol=1e-13, atol=1e-13)
+
+
 def test_fp32_error_budget():
     c = dm.case_config(0, jx=40, iy=36, kz=16)
     hs = dm.build_case(c)
