@@ -3448,7 +3448,8 @@ struct Core : CoreBase {
         const double ratio = eu > 0.0 ? std::sqrt(ec / eu) : (ec > 0.0 ? INFINITY : 0.0);
         L.st.ratio[f] = ratio;
         L.st.rel[f] = ref > 0.0 ? std::sqrt(ec / ref) : std::sqrt(ec);
-        if (!(ratio <= *worst)) *worst = ratio;
+        // a NaN ratio (a network that puts out NaN) stays the worst
+        if (!std::isnan(*worst) && !(ratio <= *worst)) *worst = ratio;
       }
       L.st.checks++;
     }
@@ -3732,7 +3733,7 @@ struct Core : CoreBase {
       L.learned_steps++;
       if (check) {
         L.st.worst_ratio = worst;
-        if (worst > L.cfg.check_tol) L.st.fallback = 1;
+        if (!(worst <= L.cfg.check_tol)) L.st.fallback = 1;  // NaN included
       }
     }
     if (prm.filterpai) {  // :333-341

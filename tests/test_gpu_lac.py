@@ -239,6 +239,38 @@ def test_failed_check_falls_back_bitwise():
     b.close()
 
 
+def test_nan_network_falls_back_bitwise():
+    """A network that puts out NaN (its last bias NaN) fails its first check
+    although NaN compares false with any tolerance; the run goes on with the
+    ordinary substeps, bit-identical to a run without the corrector."""
+    case = small_config2(48, 40, 16)
+    a, _, _ = _model(case, precision=dm.MLC_FP32)
+    b, _, _ = _model(case, precision=dm.MLC_FP32)
+    corr = Corrector(a, nhid=64, samples_per_phase=512, capacity=4096, chunk_rows=1024, check_every=1,
+                     check_tol=1.0)
+    print("case 48x40x16 fp32: NaN network output", flush=True)
+    corr.collect(2)
+    for n in (1, 2):
+        b.moloch(n)
+    corr.train(20, 128, 1e-3)
+    net = corr.network(case.kz)
+    w, bias = net.get_weights(2)
+    bias[:] = np.nan
+    net.set_weights(2, w, bias)
+    corr.mode("on")
+    for n in range(3, 6):
+        a.moloch(n)
+        b.moloch(n)
+    fa = _fields(a)
+    _same_interior(fa, _fields(b))
+    st = corr.stats()
+    print(st)
+    assert st["fallback"] and st["checks"] == 3 and st["learned_phases"] == 0
+    assert all(np.isfinite(dm.interior(fa[n])).all() for n in fa)
+    a.close()
+    b.close()
+
+
 def test_checks_pass_with_loose_tolerance():
     case = small_config2(48, 40, 16)
     a, _, _ = _model(case, precision=dm.MLC_FP32)
