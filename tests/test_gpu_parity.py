@@ -131,6 +131,16 @@ def _add_condensates(hs, c):
         hs.fields[n][:] = amp * prof * bump[None]
 
 
+def test_fp64_cold_tropopause():
+    """A 185 K tropopause: the upper third of the column is colder than
+    -75 C, where the saturation pressures of the prologue and epilogue are
+    clamped (:119-181); the other cases stay above -57 C."""
+    c = dm.case_config(0, jx=32, iy=28, kz=24, ttrop=185.0)
+    gpu, ora = _run(c, 3)
+    assert np.count_nonzero(dm.interior(ora.field("t")) < 198.15) > 0
+    _check_fp64(gpu, ora, 3, "fp64 cold tropopause")
+
+
 def test_fp64_saturated_cloudy_bubble():
     """Near-saturated air (RH 99 %) around a warm bubble, with clouds: the
     column solve's moist term (qv > 0.96 qsat where w > 0.1, :645-657) and
