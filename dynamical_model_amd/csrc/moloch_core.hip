@@ -3320,7 +3320,12 @@ struct Core : CoreBase {
     if (L.cfg.engine == 1 && (rc = mlp_pack_dev(L.net))) return rc;
     if (L.cfg.engine == 1) {
       const float *fs = L.stat + 2 * L.nin + 2 * L.nout;
-      launch(KK_LAC_GATHER, fbytes(6, 2) + (double)(L.nflp / 16) * L.fp.plane() * sizeof(__bf16),
+      // inputs: the float increments of u, v, w, pi kept by the substep
+      // kernels (inc) or u, v, w, pi and their phase-start copies, then
+      // theta and qv; the feature planes written once
+      const double in_b = inc ? (double)g.nj * g.ni * g.nk * (4.0 * sizeof(float) + 2.0 * sizeof(T))
+                              : fbytes(8, 2);
+      launch(KK_LAC_GATHER, in_b + (double)(L.nflp / 16) * L.fp.plane() * sizeof(__bf16),
              k_lac_features<T>, dim3((L.fp.fpj + 63) / 64, (L.fp.fpi + 3) / 4), dim3(256), D, lac_snap(inc), L.fp,
              L.nflp / 16, fs, fs + L.nflp);
     }
