@@ -2,6 +2,7 @@
 acoustic corrector against the FP64 core, over a long run on one GPU.
 
     python -m dynamical_model_amd.errgrowth [--config 2] [--steps 2000] [--every 100] [--variants fp32,learned]
+                                            [--max-seconds S]
 
 All handles start from the same device-built case (configurations 0-4) and
 take the same steps: ``--lead`` ordinary steps (for the learned variant:
@@ -10,7 +11,9 @@ its training-pair collection, bit-identical to ordinary steps), then
 50 steps, fallback to the ordinary loop when a check fails).  Every
 ``--every`` steps one JSON line gives, per prognostic field (u, v, w, pi,
 theta, qv), the relative L2 and the max-abs difference from the FP64 core
-over the interior, and the corrector's check statistics.  The FP64 core is
+over the interior, and the corrector's check statistics; with
+``--max-seconds`` the run ends at the first report after that much wall
+time (a last line says so).  The FP64 core is
 itself bit-identical to the FP64 CPU oracle (the parity tests), so these are
 the errors against the reference's FP64 algorithm.
 """
@@ -53,6 +56,7 @@ def main(argv=None) -> int:
     ap.add_argument("--variants", default="fp32,learned")
     ap.add_argument("--train-iters", type=int, default=6000)
     ap.add_argument("--samples", type=int, default=8192)
+    ap.add_argument("--max-seconds", type=float, default=0.0, help="stop at the first report after this wall time")
     args = ap.parse_args(argv)
 
     over = {k: getattr(args, k) for k in ("jx", "iy", "kz") if getattr(args, k)}
@@ -103,6 +107,10 @@ def main(argv=None) -> int:
                                                          "worst_ratio") if k in st}
             print(json.dumps(line), flush=True)
             del base
+            if args.max_seconds > 0 and time.perf_counter() - t0 > args.max_seconds and n < args.lead + args.steps:
+                print(json.dumps({"stopped_at_step": n, "reason": "max-seconds", "elapsed_s": time.perf_counter() - t0}),
+                      flush=True)
+                break
     ref.close()
     for m in runs.values():
         m.close()
