@@ -58,6 +58,7 @@ typedef struct mlc_lac_config {
   int chunk_rows;        /* rows per inference batch (a multiple of 256)       */
   int check_every;       /* check every this many MLC_LAC_ON steps (0: never)  */
   double check_tol;      /* fall back when a field's error ratio exceeds it    */
+                         /* or is NaN                                          */
   unsigned long long seed;
   int engine;            /* inference: 1 fused MFMA kernels reading the fields,
                           * 0 materialised features + the generic GEMM       */
