@@ -30,9 +30,22 @@
  *                                          translation equivariance (MPI halo
  *                                          order), mirror symmetry, seam
  *                                          locality of the nompi halo order
- *   parity unpinned (read, not checked): the gravity-wave term of zrom1w with
- *   its nsound factor and the moist zdth branch (:641-657), the fulleq moist
- *   coefficients 0.86/0.96/3.2/4.8 and 0.25/4.2/2.1 (:697-706).
+ *   acoustic substep phases             -> tests/test_oracle_physics_pins.py:
+ *     Equation 10, surface w               kinematic condition and metric term
+ *                                          in closed form (gzita golden)
+ *     divergence (horizontal + s part)     uniform wind over terrain is
+ *                                          divergence-free (truncation only)
+ *     Exner update, pressure gradient      discrete Lamb-wave frequency of
+ *                                          forward-backward stepping; terrain
+ *                                          PGF balances the rest state
+ *     gravity-wave term of zrom1w          anticipated interface theta from
+ *                                          the physical level heights
+ *     moist zdth branch                    latent heating of a lifted parcel
+ *                                          (Clausius-Clapeyron vs es slope)
+ *     fulleq coefficients                  moist heat capacities, to the
+ *                                          reference's two figures
+ *   tests/test_oracle_pin_mutants.py plants 18 plausible mistakes in a scratch
+ *   copy of this file; each makes the pins fail.
  */
 #include "moloch_oracle.h"
 
@@ -432,204 +445,249 @@ void ora_thomas_column(int nk, const double *zwexpl, const double *zu,
 }
 
 /* ------------------------------------------------------------------- sound
- * :483-851 */
+ * :483-851.  One substep is the sequence of the phases below (each exposed
+ * for the unit tests, which call them on constructed states). */
+
+/* surface boundary condition of w (:533-545), s(kz+1) (:547-549) and the
+ * generalised vertical velocity, Equation 10 (:553-570) */
+void ora_sound_eq10(ora_state *S) {
+  int nj = S->nj, ni = S->ni, nk = S->nk, j, i, k;
+  double *u = S->u, *v = S->v, *w = S->w, *s = S->s;
+  ORA_PAR_J
+  for (i = 1; i <= ni; i++)
+    for (j = 1; j <= nj; j++) {
+      double zuh = F3(u, j, i, nk) * F2(S->hx, j, i) + F3(u, j + 1, i, nk) * F2(S->hx, j + 1, i);
+      double zvh = F3(v, j, i, nk) * F2(S->hy, j, i) + F3(v, j, i + 1, nk) * F2(S->hy, j, i + 1);
+      F3(w, j, i, nk + 1) = 0.5 * (zuh + zvh);
+    }
+  ORA_PAR_J
+  for (i = 1; i <= ni; i++)
+    for (j = 1; j <= nj; j++) F3(s, j, i, nk + 1) = -F3(w, j, i, nk + 1);
+  ORA_PAR_IJ
+  for (k = 2; k <= nk; k++)
+    for (i = 1; i <= ni; i++)
+      for (j = 1; j <= nj; j++) {
+        double zuh = (F3(u, j, i, k) + F3(u, j, i, k - 1)) * F2(S->hx, j, i) +
+                     (F3(u, j + 1, i, k) + F3(u, j + 1, i, k - 1)) * F2(S->hx, j + 1, i);
+        double zvh = (F3(v, j, i, k) + F3(v, j, i, k - 1)) * F2(S->hy, j, i) +
+                     (F3(v, j, i + 1, k) + F3(v, j, i + 1, k - 1)) * F2(S->hy, j, i + 1);
+        F3(s, j, i, k) = -0.25 * (zuh + zvh) * S->gzitak[k - 1];
+      }
+}
+
+/* horizontal divergence, Equation 16 (:575-624) */
+void ora_sound_hdiv(ora_state *S, double dts) {
+  int nj = S->nj, ni = S->ni, nk = S->nk, j, i, k;
+  double dtrdx = dts / S->dx, dtrdy = dts / S->dx;
+  double *u = S->u, *v = S->v, *fmz = S->fmz, *zdiv2 = S->zdiv2;
+  ORA_PAR_IJ
+  for (k = 1; k <= nk; k++)
+    for (i = 1; i <= ni; i++)
+      for (j = 1; j <= nj; j++) {
+        double f0 = F3(fmz, j, i, k);
+        double zrfmzum = 2.0 / (f0 + F3(fmz, j - 1, i, k));
+        double zrfmzvm = 2.0 / (f0 + F3(fmz, j, i - 1, k));
+        double zrfmzup = 2.0 / (f0 + F3(fmz, j + 1, i, k));
+        double zrfmzvp = 2.0 / (f0 + F3(fmz, j, i + 1, k));
+        double zum, zup, zvm, zvp;
+        if (S->rotllr) {
+          zum = dtrdx * F3(u, j, i, k) * zrfmzum;
+          zup = dtrdx * F3(u, j + 1, i, k) * zrfmzup;
+          zvm = dtrdy * F3(v, j, i, k) * zrfmzvm * F2(S->rmv, j, i);
+          zvp = dtrdy * F3(v, j, i + 1, k) * zrfmzvp * F2(S->rmv, j, i + 1);
+          F3(zdiv2, j, i, k) = f0 * F2(S->mx, j, i) * ((zup - zum) + (zvp - zvm));
+        } else {
+          zum = dtrdx * F3(u, j, i, k) * F2(S->rmu, j, i) * zrfmzum;
+          zup = dtrdx * F3(u, j + 1, i, k) * F2(S->rmu, j + 1, i) * zrfmzup;
+          zvm = dtrdy * F3(v, j, i, k) * F2(S->rmv, j, i) * zrfmzvm;
+          zvp = dtrdy * F3(v, j, i + 1, k) * F2(S->rmv, j, i + 1) * zrfmzvp;
+          F3(zdiv2, j, i, k) = F2(S->mx2, j, i) * f0 * ((zup - zum) + (zvp - zvm));
+        }
+      }
+}
+
+/* vertical part of the divergence (:627-630) */
+void ora_sound_vdiv(ora_state *S, double dts) {
+  int nj = S->nj, ni = S->ni, nk = S->nk, j, i, k;
+  double dtrdz = dts / S->dzita;
+  ORA_PAR_IJ
+  for (k = 1; k <= nk; k++)
+    for (i = 1; i <= ni; i++)
+      for (j = 1; j <= nj; j++)
+        F3(S->zdiv2, j, i, k) = F3(S->zdiv2, j, i, k) +
+                                F3(S->fmz, j, i, k) * dtrdz * (F3(S->s, j, i, k) - F3(S->s, j, i, k + 1));
+}
+
+/* The coefficients of the implicit w equation of column (j,i) at substep
+ * ns (:638-676): zwexpl, zu, zd at index k-1 for k = 2..kz; zrom1w (may be
+ * NULL) receives the interface coefficient after the gravity-wave and moist
+ * terms.  Also sets deltaw(k) = -w(k) as the reference does there. */
+void ora_w_coefficients(ora_state *S, int j, int i, int ns, double dts, double *zwexpl, double *zu,
+                        double *zd, double *zrom1w_out) {
+  int nk = S->nk, k;
+  double dtrdz = dts / S->dzita;
+  double zcs2 = dtrdz * dtrdz * RDRCV;
+  double *w = S->w, *pai = S->pai, *tetav = S->tetav, *zdiv2 = S->zdiv2, *fmz = S->fmz;
+  double *fmzf = S->fmzf, *qv = QX(1);
+  for (k = nk; k >= 2; k--) {
+    double zrom1w, zqs, zdth, zwe;
+    F3(S->deltaw, j, i, k) = -F3(w, j, i, k);
+    zrom1w = 0.5 * CPD * F3(fmzf, j, i, k) * (F3(tetav, j, i, k - 1) + F3(tetav, j, i, k));
+    zrom1w = zrom1w - CPD * F3(w, j, i, k) * F3(fmzf, j, i, k) * F3(fmzf, j, i, k) * (double)ns * dtrdz *
+                          (F3(tetav, j, i, k - 1) - F3(tetav, j, i, k));
+    if (F3(qv, j, i, k) > 0.96 * F3(S->qsat, j, i, k) && F3(w, j, i, k) > 0.1) {
+      zqs = 0.5 * (F3(S->qsat, j, i, k) + F3(S->qsat, j, i, k - 1));
+      zdth = K_EGRAV * F3(w, j, i, k) * (double)(ns - 1) * dts * WLHV * WLHV * zqs /
+             (CPD * F3(pai, j, i, k - 1) * RWAT * F3(S->t, j, i, k - 1) * F3(S->t, j, i, k - 1));
+      zrom1w = zrom1w + zdth * F3(fmzf, j, i, k);
+    }
+    if (zrom1w_out) zrom1w_out[k - 1] = zrom1w;
+    zwe = F3(w, j, i, k) - zrom1w * dtrdz * (F3(pai, j, i, k - 1) - F3(pai, j, i, k)) - K_EGRAV * dts;
+    zwe = zwe + RDRCV * zrom1w * dtrdz *
+                    (F3(pai, j, i, k - 1) * F3(zdiv2, j, i, k - 1) - F3(pai, j, i, k) * F3(zdiv2, j, i, k));
+    zwexpl[k - 1] = zwe;
+    zu[k - 1] = zcs2 * F3(fmz, j, i, k - 1) * zrom1w * F3(pai, j, i, k - 1) + S->ffilt[k - 1];
+    zd[k - 1] = zcs2 * F3(fmz, j, i, k) * zrom1w * F3(pai, j, i, k) + S->ffilt[k - 1];
+  }
+}
+
+/* implicit w, Equation 19 (:634-686), one column at a time, and deltaw */
+void ora_sound_wsolve(ora_state *S, int ns, double dts) {
+  int nj = S->nj, ni = S->ni, nk = S->nk, j, i, k;
+  double zwexpl[1026], zu[1026], zd[1026], wcol[1026];
+  double *w = S->w;
+  ORA_PAR_COLUMNS
+  for (i = 1; i <= ni; i++)
+    for (j = 1; j <= nj; j++) {
+      ora_w_coefficients(S, j, i, ns, dts, zwexpl, zu, zd, NULL);
+      ora_thomas_column(nk, zwexpl, zu, zd, F3(w, j, i, 1), F3(w, j, i, nk + 1), wcol);
+      for (k = 2; k <= nk; k++) {
+        F3(w, j, i, k) = wcol[k - 1];
+        F3(S->deltaw, j, i, k) = F3(S->deltaw, j, i, k) + F3(w, j, i, k);
+      }
+    }
+}
+
+/* The fulleq moist corrections of one point (:697-706): zdiv2 scaled by the
+ * moist factor, then tetav from the scaled zdiv2. */
+void ora_fulleq_point(double qv, double qc, double qwltot, double qwitot, double *zdiv2, double *tetav) {
+  *zdiv2 = *zdiv2 * (1.0 + 0.86 * qv + 3.2 * qc) / (1.0 + 0.96 * qv + 4.8 * qc);
+  *tetav = *tetav * (1.0 + RDRCV * *zdiv2 * (0.25 * qv + 4.2 * qwltot + 2.1 * qwitot));
+}
+
+/* w part of the divergence, the fulleq corrections, the divergence filter,
+ * the halos and the new Exner function (:690-728) */
+void ora_sound_exner(ora_state *S, double dts) {
+  int nj = S->nj, ni = S->ni, nk = S->nk, j, i, k;
+  double dtrdz = dts / S->dzita;
+  double *w = S->w, *pai = S->pai, *zdiv2 = S->zdiv2;
+  double *qv = QX(1), *qc = QX(2);
+  ORA_PAR_IJ
+  for (k = 1; k <= nk; k++)
+    for (i = 1; i <= ni; i++)
+      for (j = 1; j <= nj; j++)
+        F3(zdiv2, j, i, k) = F3(zdiv2, j, i, k) + dtrdz * F3(S->fmz, j, i, k) * (F3(w, j, i, k) - F3(w, j, i, k + 1));
+
+  if (S->do_fulleq) { /* :695-712 */
+    ORA_PAR_IJ
+    for (k = 1; k <= nk; k++)
+      for (i = 1; i <= ni; i++)
+        for (j = 1; j <= nj; j++) {
+          long x = I3(j, i, k);
+          ora_fulleq_point(qv[x], qc[x], S->qwltot[x], S->qwitot[x], &zdiv2[x], &S->tetav[x]);
+        }
+    halo_faces1(S, S->tetav, nk);
+  }
+
+  if (S->divfilter) ora_divergence_filter(S); /* :714 */
+
+  if (!S->halo_mpi) { /* nompi: halos taken before the Exner update (:716-723) */
+    halo_faces1(S, pai, nk);
+    halo_faces1(S, S->deltaw, nk + 1);
+  }
+  ORA_PAR_IJ
+  for (k = 1; k <= nk; k++) /* :726-728 */
+    for (i = 1; i <= ni; i++)
+      for (j = 1; j <= nj; j++) F3(pai, j, i, k) = F3(pai, j, i, k) * (1.0 - RDRCV * F3(zdiv2, j, i, k));
+  if (S->halo_mpi) { /* MPI tree order, mod_moloch.F90:697-703 */
+    halo_faces1(S, pai, nk);
+    halo_faces1(S, S->deltaw, nk + 1);
+  }
+}
+
+/* snapshots including halos (:730-735) and the horizontal momentum,
+ * Equations 17 and 18 (:737-838) */
+void ora_sound_momentum(ora_state *S, double dts) {
+  int nj = S->nj, ni = S->ni, nk = S->nk, j, i, k;
+  double dtrdx = dts / S->dx, dtrdy = dts / S->dx;
+  double *u = S->u, *v = S->v, *pai = S->pai, *tetav = S->tetav, *deltaw = S->deltaw;
+  ORA_PAR_IJ
+  for (k = 1; k <= nk; k++)
+    for (i = 0; i <= ni + 1; i++)
+      for (j = 0; j <= nj + 1; j++) {
+        F3(S->ud, j, i, k) = F3(u, j, i, k);
+        F3(S->vd, j, i, k) = F3(v, j, i, k);
+      }
+  ORA_PAR_IJ
+  for (k = 1; k <= nk; k++)
+    for (i = 1; i <= ni; i++)
+      for (j = 1; j <= nj; j++) {
+        double zcx = dtrdx * F2(S->mu, j, i);
+        double zfz = K_EGRAV * dts + 0.25 * (F3(deltaw, j - 1, i, k) + F3(deltaw, j - 1, i, k + 1) +
+                                             F3(deltaw, j, i, k) + F3(deltaw, j, i, k + 1));
+        double zrom1u = 0.5 * CPD * (F3(tetav, j - 1, i, k) + F3(tetav, j, i, k));
+        double zcor1u = F2(S->coru, j, i) * dts * 0.25 *
+                        (F3(S->vd, j, i, k) + F3(S->vd, j - 1, i, k) + F3(S->vd, j - 1, i + 1, k) +
+                         F3(S->vd, j, i + 1, k));
+        F3(u, j, i, k) = F3(u, j, i, k) + zcor1u - zfz * F2(S->hx, j, i) * S->gzitakh[k - 1] -
+                         zcx * zrom1u * (F3(pai, j, i, k) - F3(pai, j - 1, i, k));
+      }
+  ORA_PAR_IJ
+  for (k = 1; k <= nk; k++)
+    for (i = 1; i <= ni; i++)
+      for (j = 1; j <= nj; j++) {
+        double zcy = S->rotllr ? dtrdy : dtrdy * F2(S->mv, j, i);
+        double zfz = K_EGRAV * dts + 0.25 * (F3(deltaw, j, i - 1, k) + F3(deltaw, j, i - 1, k + 1) +
+                                             F3(deltaw, j, i, k) + F3(deltaw, j, i, k + 1));
+        double zrom1v = 0.5 * CPD * (F3(tetav, j, i - 1, k) + F3(tetav, j, i, k));
+        double zcor1v = F2(S->corv, j, i) * dts * 0.25 *
+                        (F3(S->ud, j, i, k) + F3(S->ud, j, i - 1, k) + F3(S->ud, j + 1, i, k) +
+                         F3(S->ud, j + 1, i - 1, k));
+        F3(v, j, i, k) = F3(v, j, i, k) - zcor1v - zfz * F2(S->hy, j, i) * S->gzitakh[k - 1] -
+                         zcy * zrom1v * (F3(pai, j, i, k) - F3(pai, j, i - 1, k));
+      }
+}
+
+/* One acoustic substep ns of length dts (:512-838). */
+void ora_substep(ora_state *S, int ns, double dts) {
+  halo_corners1(S, S->u); /* :514-521 */
+  halo_corners1(S, S->v); /* :522-529 */
+  ora_sound_eq10(S);
+  ora_sound_hdiv(S, dts);
+  if (S->do_divdamp) ora_divdamp(S, dts); /* :626 */
+  ora_sound_vdiv(S, dts);
+  ora_sound_wsolve(S, ns, dts);
+  ora_sound_exner(S, dts);
+  ora_sound_momentum(S, dts);
+}
+
 void ora_sound(ora_state *S, double dts) {
   int nj = S->nj, ni = S->ni, nk = S->nk, j, i, k, ns;
-  double dtrdx = dts / S->dx, dtrdy = dts / S->dx, dtrdz = dts / S->dzita;
-  double zcs2 = dtrdz * dtrdz * RDRCV;
-  double *u = S->u, *v = S->v, *w = S->w, *pai = S->pai, *tetav = S->tetav;
-  double *zdiv2 = S->zdiv2, *s = S->s, *deltaw = S->deltaw, *fmz = S->fmz;
-  double *fmzf = S->fmzf;
-  double *qv = QX(1), *qc = QX(2);
-  double zwexpl[1026], zu[1026], zd[1026], wcol[1026];
+  double *s = S->s, *w = S->w;
 
   ORA_PAR_J
   for (i = 1; i <= ni; i++)
     for (j = 1; j <= nj; j++) F3(S->wwkw, j, i, nk + 1) = 0.0;
 
-  if (!S->do_fulleq) halo_faces1(S, tetav, nk);
+  if (!S->do_fulleq) halo_faces1(S, S->tetav, nk);
 
-  for (ns = 1; ns <= S->nsound; ns++) {
-    halo_corners1(S, u); /* :514-521 */
-    halo_corners1(S, v); /* :522-529 */
-
-    /* surface boundary condition of w (:533-545) and s (:547-549) */
-    ORA_PAR_J
-    for (i = 1; i <= ni; i++)
-      for (j = 1; j <= nj; j++) {
-        double zuh = F3(u, j, i, nk) * F2(S->hx, j, i) + F3(u, j + 1, i, nk) * F2(S->hx, j + 1, i);
-        double zvh = F3(v, j, i, nk) * F2(S->hy, j, i) + F3(v, j, i + 1, nk) * F2(S->hy, j, i + 1);
-        F3(w, j, i, nk + 1) = 0.5 * (zuh + zvh);
-      }
-    ORA_PAR_J
-    for (i = 1; i <= ni; i++)
-      for (j = 1; j <= nj; j++) F3(s, j, i, nk + 1) = -F3(w, j, i, nk + 1);
-
-    /* generalised vertical velocity, Equation 10 (:553-570) */
-    ORA_PAR_IJ
-    for (k = 2; k <= nk; k++)
-      for (i = 1; i <= ni; i++)
-        for (j = 1; j <= nj; j++) {
-          double zuh = (F3(u, j, i, k) + F3(u, j, i, k - 1)) * F2(S->hx, j, i) +
-                       (F3(u, j + 1, i, k) + F3(u, j + 1, i, k - 1)) * F2(S->hx, j + 1, i);
-          double zvh = (F3(v, j, i, k) + F3(v, j, i, k - 1)) * F2(S->hy, j, i) +
-                       (F3(v, j, i + 1, k) + F3(v, j, i + 1, k - 1)) * F2(S->hy, j, i + 1);
-          F3(s, j, i, k) = -0.25 * (zuh + zvh) * S->gzitak[k - 1];
-        }
-
-    /* horizontal divergence, Equation 16 (:575-624) */
-    ORA_PAR_IJ
-    for (k = 1; k <= nk; k++)
-      for (i = 1; i <= ni; i++)
-        for (j = 1; j <= nj; j++) {
-          double f0 = F3(fmz, j, i, k);
-          double zrfmzum = 2.0 / (f0 + F3(fmz, j - 1, i, k));
-          double zrfmzvm = 2.0 / (f0 + F3(fmz, j, i - 1, k));
-          double zrfmzup = 2.0 / (f0 + F3(fmz, j + 1, i, k));
-          double zrfmzvp = 2.0 / (f0 + F3(fmz, j, i + 1, k));
-          double zum, zup, zvm, zvp;
-          if (S->rotllr) {
-            zum = dtrdx * F3(u, j, i, k) * zrfmzum;
-            zup = dtrdx * F3(u, j + 1, i, k) * zrfmzup;
-            zvm = dtrdy * F3(v, j, i, k) * zrfmzvm * F2(S->rmv, j, i);
-            zvp = dtrdy * F3(v, j, i + 1, k) * zrfmzvp * F2(S->rmv, j, i + 1);
-            F3(zdiv2, j, i, k) = f0 * F2(S->mx, j, i) * ((zup - zum) + (zvp - zvm));
-          } else {
-            zum = dtrdx * F3(u, j, i, k) * F2(S->rmu, j, i) * zrfmzum;
-            zup = dtrdx * F3(u, j + 1, i, k) * F2(S->rmu, j + 1, i) * zrfmzup;
-            zvm = dtrdy * F3(v, j, i, k) * F2(S->rmv, j, i) * zrfmzvm;
-            zvp = dtrdy * F3(v, j, i + 1, k) * F2(S->rmv, j, i + 1) * zrfmzvp;
-            F3(zdiv2, j, i, k) = F2(S->mx2, j, i) * f0 * ((zup - zum) + (zvp - zvm));
-          }
-        }
-
-    if (S->do_divdamp) ora_divdamp(S, dts); /* :626 */
-
-    /* vertical part of the divergence (:627-630) */
-    ORA_PAR_IJ
-    for (k = 1; k <= nk; k++)
-      for (i = 1; i <= ni; i++)
-        for (j = 1; j <= nj; j++)
-          F3(zdiv2, j, i, k) = F3(zdiv2, j, i, k) +
-                               F3(fmz, j, i, k) * dtrdz * (F3(s, j, i, k) - F3(s, j, i, k + 1));
-
-    /* implicit w, Equation 19 (:634-686), one column at a time */
-    ORA_PAR_COLUMNS
-    for (i = 1; i <= ni; i++)
-      for (j = 1; j <= nj; j++) {
-        for (k = nk; k >= 2; k--) {
-          double zrom1w, zqs, zdth, zwe;
-          F3(deltaw, j, i, k) = -F3(w, j, i, k);
-          zrom1w = 0.5 * CPD * F3(fmzf, j, i, k) * (F3(tetav, j, i, k - 1) + F3(tetav, j, i, k));
-          zrom1w = zrom1w - CPD * F3(w, j, i, k) * F3(fmzf, j, i, k) * F3(fmzf, j, i, k) *
-                                (double)ns * dtrdz *
-                                (F3(tetav, j, i, k - 1) - F3(tetav, j, i, k));
-          if (F3(qv, j, i, k) > 0.96 * F3(S->qsat, j, i, k) && F3(w, j, i, k) > 0.1) {
-            zqs = 0.5 * (F3(S->qsat, j, i, k) + F3(S->qsat, j, i, k - 1));
-            zdth = K_EGRAV * F3(w, j, i, k) * (double)(ns - 1) * dts * WLHV * WLHV * zqs /
-                   (CPD * F3(pai, j, i, k - 1) * RWAT * F3(S->t, j, i, k - 1) * F3(S->t, j, i, k - 1));
-            zrom1w = zrom1w + zdth * F3(fmzf, j, i, k);
-          }
-          zwe = F3(w, j, i, k) - zrom1w * dtrdz * (F3(pai, j, i, k - 1) - F3(pai, j, i, k)) -
-                K_EGRAV * dts;
-          zwe = zwe + RDRCV * zrom1w * dtrdz *
-                          (F3(pai, j, i, k - 1) * F3(zdiv2, j, i, k - 1) -
-                           F3(pai, j, i, k) * F3(zdiv2, j, i, k));
-          zwexpl[k - 1] = zwe;
-          zu[k - 1] = zcs2 * F3(fmz, j, i, k - 1) * zrom1w * F3(pai, j, i, k - 1) + S->ffilt[k - 1];
-          zd[k - 1] = zcs2 * F3(fmz, j, i, k) * zrom1w * F3(pai, j, i, k) + S->ffilt[k - 1];
-        }
-        ora_thomas_column(nk, zwexpl, zu, zd, F3(w, j, i, 1), F3(w, j, i, nk + 1), wcol);
-        for (k = 2; k <= nk; k++) {
-          F3(w, j, i, k) = wcol[k - 1];
-          F3(deltaw, j, i, k) = F3(deltaw, j, i, k) + F3(w, j, i, k);
-        }
-      }
-
-    /* new Exner function, Equation 19 (:690-693) */
-    ORA_PAR_IJ
-    for (k = 1; k <= nk; k++)
-      for (i = 1; i <= ni; i++)
-        for (j = 1; j <= nj; j++)
-          F3(zdiv2, j, i, k) = F3(zdiv2, j, i, k) +
-                               dtrdz * F3(fmz, j, i, k) * (F3(w, j, i, k) - F3(w, j, i, k + 1));
-
-    if (S->do_fulleq) { /* :695-712 */
-      ORA_PAR_IJ
-      for (k = 1; k <= nk; k++)
-        for (i = 1; i <= ni; i++)
-          for (j = 1; j <= nj; j++) {
-            long x = I3(j, i, k);
-            zdiv2[x] = zdiv2[x] * (1.0 + 0.86 * qv[x] + 3.2 * qc[x]) /
-                       (1.0 + 0.96 * qv[x] + 4.8 * qc[x]);
-            tetav[x] = tetav[x] * (1.0 + RDRCV * zdiv2[x] *
-                                             (0.25 * qv[x] + 4.2 * S->qwltot[x] + 2.1 * S->qwitot[x]));
-          }
-      halo_faces1(S, tetav, nk);
-    }
-
-    if (S->divfilter) ora_divergence_filter(S); /* :714 */
-
-    if (!S->halo_mpi) { /* nompi: halos taken before the Exner update (:716-723) */
-      halo_faces1(S, pai, nk);
-      halo_faces1(S, deltaw, nk + 1);
-    }
-    ORA_PAR_IJ
-    for (k = 1; k <= nk; k++) /* :726-728 */
-      for (i = 1; i <= ni; i++)
-        for (j = 1; j <= nj; j++)
-          F3(pai, j, i, k) = F3(pai, j, i, k) * (1.0 - RDRCV * F3(zdiv2, j, i, k));
-    if (S->halo_mpi) { /* MPI tree order, mod_moloch.F90:697-703 */
-      halo_faces1(S, pai, nk);
-      halo_faces1(S, deltaw, nk + 1);
-    }
-
-    /* snapshots including halos (:730-735) */
-    ORA_PAR_IJ
-    for (k = 1; k <= nk; k++)
-      for (i = 0; i <= ni + 1; i++)
-        for (j = 0; j <= nj + 1; j++) {
-          F3(S->ud, j, i, k) = F3(u, j, i, k);
-          F3(S->vd, j, i, k) = F3(v, j, i, k);
-        }
-
-    /* horizontal momentum, Equations 17 and 18 (:737-838) */
-    ORA_PAR_IJ
-    for (k = 1; k <= nk; k++)
-      for (i = 1; i <= ni; i++)
-        for (j = 1; j <= nj; j++) {
-          double zcx = dtrdx * F2(S->mu, j, i);
-          double zfz = K_EGRAV * dts + 0.25 * (F3(deltaw, j - 1, i, k) + F3(deltaw, j - 1, i, k + 1) +
-                                               F3(deltaw, j, i, k) + F3(deltaw, j, i, k + 1));
-          double zrom1u = 0.5 * CPD * (F3(tetav, j - 1, i, k) + F3(tetav, j, i, k));
-          double zcor1u = F2(S->coru, j, i) * dts * 0.25 *
-                          (F3(S->vd, j, i, k) + F3(S->vd, j - 1, i, k) + F3(S->vd, j - 1, i + 1, k) +
-                           F3(S->vd, j, i + 1, k));
-          F3(u, j, i, k) = F3(u, j, i, k) + zcor1u - zfz * F2(S->hx, j, i) * S->gzitakh[k - 1] -
-                           zcx * zrom1u * (F3(pai, j, i, k) - F3(pai, j - 1, i, k));
-        }
-    ORA_PAR_IJ
-    for (k = 1; k <= nk; k++)
-      for (i = 1; i <= ni; i++)
-        for (j = 1; j <= nj; j++) {
-          double zcy = S->rotllr ? dtrdy : dtrdy * F2(S->mv, j, i);
-          double zfz = K_EGRAV * dts + 0.25 * (F3(deltaw, j, i - 1, k) + F3(deltaw, j, i - 1, k + 1) +
-                                               F3(deltaw, j, i, k) + F3(deltaw, j, i, k + 1));
-          double zrom1v = 0.5 * CPD * (F3(tetav, j, i - 1, k) + F3(tetav, j, i, k));
-          double zcor1v = F2(S->corv, j, i) * dts * 0.25 *
-                          (F3(S->ud, j, i, k) + F3(S->ud, j, i - 1, k) + F3(S->ud, j + 1, i, k) +
-                           F3(S->ud, j + 1, i - 1, k));
-          F3(v, j, i, k) = F3(v, j, i, k) - zcor1v - zfz * F2(S->hy, j, i) * S->gzitakh[k - 1] -
-                           zcy * zrom1v * (F3(pai, j, i, k) - F3(pai, j, i - 1, k));
-        }
-  } /* sound loop */
+  for (ns = 1; ns <= S->nsound; ns++) ora_substep(S, ns, dts);
 
   /* complete Equation 10 (:843-849) */
   ORA_PAR_IJ
   for (k = 2; k <= nk; k++)
     for (i = 1; i <= ni; i++)
-      for (j = 1; j <= nj; j++)
-        F3(s, j, i, k) = (F3(w, j, i, k) + F3(s, j, i, k)) * F3(fmzf, j, i, k);
+      for (j = 1; j <= nj; j++) F3(s, j, i, k) = (F3(w, j, i, k) + F3(s, j, i, k)) * F3(S->fmzf, j, i, k);
   ORA_PAR_J
   for (i = 1; i <= ni; i++)
     for (j = 1; j <= nj; j++) {

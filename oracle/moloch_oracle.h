@@ -82,6 +82,23 @@ void ora_init_moloch(ora_state *S);
 void ora_reset_tendencies(ora_state *S);            /* :1313-1325 */
 void ora_prologue(ora_state *S);                    /* :291-323 */
 void ora_sound(ora_state *S, double dts);           /* :483-851 */
+/* One substep ns of ora_sound (:512-838) and its phases, in order:
+ * halos of u and v, then */
+void ora_substep(ora_state *S, int ns, double dts);
+void ora_sound_eq10(ora_state *S);                  /* :533-570 */
+void ora_sound_hdiv(ora_state *S, double dts);      /* :575-624 */
+/* (ora_divdamp when do_divdamp) */
+void ora_sound_vdiv(ora_state *S, double dts);      /* :627-630 */
+void ora_sound_wsolve(ora_state *S, int ns, double dts); /* :634-686 */
+void ora_sound_exner(ora_state *S, double dts);     /* :690-728 */
+void ora_sound_momentum(ora_state *S, double dts);  /* :730-838 */
+/* Coefficients of the implicit w equation of one column (:638-676), as
+ * ora_sound_wsolve builds them: zwexpl, zu, zd (and zrom1w if not NULL) at
+ * index k-1 for k = 2..kz. */
+void ora_w_coefficients(ora_state *S, int j, int i, int ns, double dts, double *zwexpl, double *zu,
+                        double *zd, double *zrom1w);
+/* The fulleq moist corrections of one point (:697-706). */
+void ora_fulleq_point(double qv, double qc, double qwltot, double qwitot, double *zdiv2, double *tetav);
 void ora_divdamp(ora_state *S, double dts);         /* :853-895 */
 void ora_advection(ora_state *S, double dta);       /* :897-927 */
 void ora_wafone(ora_state *S, double *pp, double dta); /* :929-1311 */

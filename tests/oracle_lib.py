@@ -63,6 +63,9 @@ def build_oracle() -> None:
 def lib(threaded: bool = False) -> C.CDLL:
     if threaded not in _libs:
         path = ORACLE_LIB_OMP if threaded else ORACLE_LIB
+        # a scratch build of the oracle with a planted mistake (the mutation
+        # test of the pins runs the pins against it in a child process)
+        path = os.environ.get("MLC_ORACLE_MUTANT", path)
         if not os.path.exists(path):
             build_oracle()
         L = C.CDLL(path)
@@ -76,6 +79,12 @@ def lib(threaded: bool = False) -> C.CDLL:
             "ora_divergence_filter": [P], "ora_moloch": [P, C.c_int, C.c_int],
             "ora_thomas_column": [C.c_int, _D, _D, _D, C.c_double, C.c_double, _D],
             "ora_get_constants": [C.POINTER(OraConstants)],
+            "ora_substep": [P, C.c_int, C.c_double], "ora_sound_eq10": [P],
+            "ora_sound_hdiv": [P, C.c_double], "ora_sound_vdiv": [P, C.c_double],
+            "ora_sound_wsolve": [P, C.c_int, C.c_double], "ora_sound_exner": [P, C.c_double],
+            "ora_sound_momentum": [P, C.c_double],
+            "ora_w_coefficients": [P, C.c_int, C.c_int, C.c_int, C.c_double, _D, _D, _D, _D],
+            "ora_fulleq_point": [C.c_double, C.c_double, C.c_double, C.c_double, _D, _D],
         }.items():
             f = getattr(L, name)
             f.argtypes = args
