@@ -160,7 +160,9 @@ def rel_l2(a: np.ndarray, b: np.ndarray) -> float:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=50,
+                    help="timed steps; 50 = one corrector check period, so the timed window carries its "
+                         "steady-state share of checks (and of retraining after a failed check)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=4, help="benchmark configuration (grid and physics)")
     ap.add_argument("--jx", type=int, default=0)
@@ -179,6 +181,12 @@ def main() -> int:
     ap.add_argument("--variants", default="fp64,fp32,learned")
     ap.add_argument("--check-tol", type=float, default=1.0,
                     help="corrector check: fall back when a field's error ratio exceeds this")
+    ap.add_argument("--rel-tol", type=float, default=1e-3,
+                    help="corrector check: fall back when a field's relative L2 error against the ordinary "
+                         "substeps exceeds this (0: no error budget)")
+    ap.add_argument("--retrain-steps", type=int, default=2,
+                    help="after a failed check: ordinary steps collecting fresh pairs before retraining "
+                         "(0: stay on the ordinary loop)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -196,7 +204,9 @@ def main() -> int:
     learned_cfg = {
         "config": dict(nhid=512, samples_per_phase=args.samples,
                        capacity=max(131072, 3 * args.lead * args.samples), chunk_rows=args.chunk, check_every=50,
-                       check_tol=args.check_tol, seed=11, engine=1),
+                       check_tol=args.check_tol, seed=11, engine=1, rel_tol=args.rel_tol,
+                       retrain_steps=args.retrain_steps, retrain_iters=max(1, args.train_iters // 4),
+                       retrain_batch=args.train_batch, retrain_lr=3e-4, max_retrains=3),
         "batch": args.train_batch,
         "schedule": [(args.train_iters // 2, 1e-3), (args.train_iters // 4, 3e-4), (args.train_iters // 4, 1e-4)],
     }
@@ -247,6 +257,14 @@ def main() -> int:
     stencils = {k: roofline_of(chosen["_prof"], k, traffic) for k in chosen["_prof"]
                 if k not in MFMA_KINDS and chosen["_prof"][k][2] > 0}
     mlp = roofline_of(chosen["_prof"], "lac_mlp", traffic) if "lac_mlp" in chosen["_prof"] else None
+    # the rate of plain device-to-device copies (the corrector's phase-start
+    # snapshots, hipMemcpyAsync) in the same timed window: what streaming
+    # reaches on this box, next to the 8 TB/s peak
+    # (the kind is charged the interior; a copy moves whole padded planes of
+    # pj = roundup(jx + 34, 32) by iy + 4 points)
+    copy = chosen["_prof"].get("lac_copy")
+    plane = ((jx + 34 + 31) // 32 * 32) * (iy + 4)
+    copy_gbs = copy[2] * plane / (jx * iy) / (copy[0] * 1e-3) / 1e9 if copy and copy[0] > 0 else None
 
     cpu = None
     serial = None
@@ -304,6 +322,7 @@ def main() -> int:
         "stencil_kernels": {k: {x: r[x] for x in ("achieved", "frac", "traffic", "avg_launch_ms", "measured_gbs")
                                 if x in r} for k, r in stencils.items()},
         "mlp_roofline": mlp,
+        "copy_gbs": copy_gbs,
         "cpu_baseline": cpu,
         "variants": results,
     }

@@ -85,6 +85,8 @@ class LacConfig(C.Structure):
         ("nhid", C.c_int), ("samples_per_phase", C.c_int), ("capacity", C.c_int),
         ("chunk_rows", C.c_int), ("check_every", C.c_int), ("check_tol", C.c_double),
         ("seed", C.c_ulonglong), ("engine", C.c_int),
+        ("rel_tol", C.c_double), ("retrain_steps", C.c_int), ("retrain_iters", C.c_int),
+        ("retrain_batch", C.c_int), ("retrain_lr", C.c_double), ("max_retrains", C.c_int),
     ]
 
 
@@ -93,7 +95,8 @@ class LacStats(C.Structure):
         ("samples", C.c_longlong), ("learned_phases", C.c_longlong),
         ("plain_phases", C.c_longlong), ("checks", C.c_longlong), ("fallback", C.c_int),
         ("last_loss", C.c_double), ("ratio", C.c_double * 4), ("rel", C.c_double * 4),
-        ("worst_ratio", C.c_double),
+        ("worst_ratio", C.c_double), ("worst_rel", C.c_double), ("retrains", C.c_longlong),
+        ("collect_phases", C.c_longlong), ("failed_checks", C.c_longlong),
     ]
 
 

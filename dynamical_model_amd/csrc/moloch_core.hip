@@ -111,16 +111,20 @@ template <typename T>
 struct Dev {
   Geo g;
   // state
-  T *u, *v, *un, *vn, *w, *t, *tetav, *qv, *qc, *qi, *qr, *qs;
+  T *u, *v, *un, *vn, *w, *qv, *qc, *qi, *qr, *qs;
   T *tvirt, *p, *rho, *qsat, *ux, *vx, *ps;
-  double *pai;
+  // the Exner function, theta and the temperature stay FP64 in both
+  // precisions: the hydrostatic balance of the w equation cancels theta
+  // times the Exner difference against g dts, which amplifies a rounding of
+  // theta by g dts / |w|
+  double *pai, *t, *tetav;
   // geometry
   T *fmz, *fmzf, *zeta, *hx, *hy, *mu, *mv, *mx, *rmu, *rmv, *mx2, *coru, *corv;
   const double *gzitak, *gzitakh, *xknu, *ffilt;
   // scratch
-  T *s, *deltaw, *zdh, *zds, *zdf, *wx, *qwltot, *qwitot, *tf, *sm;
+  T *s, *deltaw, *zdh, *zds, *zdf, *wx, *qwltot, *qwitot;
   double *wz, *p0;        // advection scratch, sized for an FP64 field
-  double *pf, *smd;       // filter scratch
+  double *pf, *smd, *tf, *smt;  // filter scratch
   double *thg, *thc;      // tridiagonal elimination (FP64)
   double *diag;           // reduction partials
   // increments of the substep being run (its result minus its start) of u,
@@ -248,7 +252,7 @@ __global__ void __launch_bounds__(256) k_prologue(Dev<T> D, Par P, int do_pf, in
       D.qwltot[x] = qc + qr;
       D.qwitot[x] = qi + qs;
     }
-    T tet = (T)(tv / pai);
+    double tet = tv / pai;
     D.tetav[x] = tet;
     if (do_pf) D.pf[x] = pai;
     if (do_tf) D.tf[x] = tet;
@@ -402,7 +406,9 @@ __global__ void __launch_bounds__(256) k_sound_b(Dev<T> D, Par P, double dts, in
     T qv = D.qv[x], qc = D.qc[x];
     if (P.fulleq) {
       zf = zf * (T(1) + T(0.86) * qv + T(3.2) * qc) / (T(1) + T(0.96) * qv + T(4.8) * qc);
-      T tet = D.tetav[x] * (T(1) + (T)kRdrcv * zf * (T(0.25) * qv + T(4.2) * D.qwltot[x] + T(2.1) * D.qwitot[x]));
+      // the increment in the storage type, 1 + increment and the product in FP64
+      double tet = D.tetav[x] *
+                   (1.0 + (double)((T)kRdrcv * zf * (T(0.25) * qv + T(4.2) * D.qwltot[x] + T(2.1) * D.qwitot[x])));
       D.tetav[x] = tet;
       store_faces(D.tetav, g, j, i, k, tet);
     }
@@ -458,7 +464,8 @@ __global__ void __launch_bounds__(NS * CW) k_sound_bs(Dev<T> D, Par P, double dt
   // theta, old w, qv and fmz of each lane's levels, kept from the first
   // phase to the last (in registers they would cost occupancy, read again
   // from memory they would cost traffic)
-  __shared__ T stash[4][L][NS * CW];
+  __shared__ T stash[3][L][NS * CW];  // old w, qv, fmz
+  __shared__ double sth[L][NS * CW];
   const int tl = threadIdx.y * 64 + threadIdx.x;
   const Geo &g = D.g;
   const int lane = threadIdx.x % CW;                           // column within the block
@@ -498,8 +505,8 @@ __global__ void __launch_bounds__(NS * CW) k_sound_bs(Dev<T> D, Par P, double dt
   T zz[L];
   double pp[L];
   // level a-1 (the segment's upper neighbour; level 1 for wave 0)
-  T zt = 0, tht = 0, fzt = 0;
-  double ppt = 0;
+  T zt = 0, fzt = 0;
+  double ppt = 0, tht = 0;
   if (cnt > 0) {
     {
       const long x = g.at(j, i, a - 1);
@@ -516,11 +523,12 @@ __global__ void __launch_bounds__(NS * CW) k_sound_bs(Dev<T> D, Par P, double dt
       const long xk = g.at(j, i, k), xm = xk - g.plane;
       const T zkt = level_z(xk);
       zz[l] = zkt;
-      const T tkt = D.tetav[xk], wkt = D.w[xk], qvt = D.qv[xk], fzt_k = D.fmz[xk];
-      stash[0][l][tl] = tkt;
-      stash[1][l][tl] = wkt;
-      stash[2][l][tl] = qvt;
-      stash[3][l][tl] = fzt_k;
+      const double tkt = D.tetav[xk];
+      const T wkt = D.w[xk], qvt = D.qv[xk], fzt_k = D.fmz[xk];
+      sth[l][tl] = tkt;
+      stash[0][l][tl] = wkt;
+      stash[1][l][tl] = qvt;
+      stash[2][l][tl] = fzt_k;
       pp[l] = D.pai[xk];
       const double zk = (double)zkt, tk = (double)tkt, pk = pp[l], fzk = (double)fzt_k;
       const double qsk = (double)D.qsat[xk];
@@ -597,12 +605,14 @@ __global__ void __launch_bounds__(NS * CW) k_sound_bs(Dev<T> D, Par P, double dt
   if (cnt == 0) return;
 
   // w, deltaw; theta and pi of each level once w above and below are known
-  auto finish = [&](long x, int k, T zk, T fzk, T thk, T qv, double po, T wtop, T wbot) {
+  auto finish = [&](long x, int k, T zk, T fzk, double thk, T qv, double po, T wtop, T wbot) {
     T zf = zk + dtrdz_t * fzk * (wtop - wbot);
     T qc = D.qc[x];
     if (P.fulleq) {
       zf = zf * (T(1) + T(0.86) * qv + T(3.2) * qc) / (T(1) + T(0.96) * qv + T(4.8) * qc);
-      T tet = thk * (T(1) + (T)kRdrcv * zf * (T(0.25) * qv + T(4.2) * D.qwltot[x] + T(2.1) * D.qwitot[x]));
+      // the increment in the storage type, 1 + increment and the product in FP64
+      double tet =
+          thk * (1.0 + (double)((T)kRdrcv * zf * (T(0.25) * qv + T(4.2) * D.qwltot[x] + T(2.1) * D.qwitot[x])));
       D.tetav[x] = tet;
       store_faces(D.tetav, g, j, i, k, tet);
     }
@@ -625,13 +635,13 @@ __global__ void __launch_bounds__(NS * CW) k_sound_bs(Dev<T> D, Par P, double dt
       const int k = a + l;
       const long x = g.at(j, i, k);
       const T wn_t = (T)rz[l];
-      T dw = -stash[1][l][tl] + wn_t;
+      T dw = -stash[0][l][tl] + wn_t;
       D.w[x] = wn_t;
       D.deltaw[x] = dw;
       if (D.iw) D.iw[x - g.plane] = (float)dw;
       store_faces(D.deltaw, g, j, i, k, dw);
       const T wb = l + 1 < cnt ? (T)rz[l + 1 < L ? l + 1 : l] : wnext;
-      finish(x, k, zz[l], stash[3][l][tl], stash[0][l][tl], stash[2][l][tl], pp[l], wn_t, wb);
+      finish(x, k, zz[l], stash[2][l][tl], sth[l][tl], stash[1][l][tl], pp[l], wn_t, wb);
     }
   }
 }
@@ -1500,18 +1510,18 @@ __global__ void __launch_bounds__(256) k_epilogue(Dev<T> D) {
   for (int k = 1; k <= nk; k++) {
     long x = g.at(j, i, k);
     double pai = D.pai[x];
-    double tv = (double)D.tetav[x] * pai;
+    double tv = D.tetav[x] * pai;
     T tv_t = (T)tv;
     D.tvirt[x] = tv_t;
-    double t = (double)tv_t / (1.0 + kEp1 * (double)D.qv[x] - (double)D.qc[x] - (double)D.qi[x] - (double)D.qr[x] -
-                               (double)D.qs[x]);
-    T t_t = (T)t;
-    D.t[x] = t_t;
+    // the temperature from the FP64 virtual temperature (tv_t == tv in FP64)
+    double t = tv / (1.0 + kEp1 * (double)D.qv[x] - (double)D.qc[x] - (double)D.qi[x] - (double)D.qr[x] -
+                     (double)D.qs[x]);
+    D.t[x] = t;
     double p = pow(pai, kCpovr) * kP00;
     T p_t = (T)p;
     D.p[x] = p_t;
-    D.rho[x] = (T)((double)p_t / (kRgas * (double)t_t));
-    D.qsat[x] = (T)d_pfwsat((double)t_t, (double)p_t);
+    D.rho[x] = (T)((double)p_t / (kRgas * t));
+    D.qsat[x] = (T)d_pfwsat(t, (double)p_t);
     if (k == nk - 1) tvkm = (double)tv_t;
     if (k == nk) {
       tvk = (double)tv_t;
@@ -2541,7 +2551,7 @@ __global__ void __launch_bounds__(256) k_case_build(Dev<T> D, CaseDev c) {
         }
       }
     }
-    D.t[x] = (T)t;
+    D.t[x] = t;
   }
 }
 
@@ -2619,6 +2629,7 @@ struct Core : CoreBase {
   bool initialised = false;
   bool unfused_advection = false;  // params.unfused: the three-sweep wafone kernels
   T *gwz[MAXF] = {}, *gout[MAXF] = {};  // scratch of the grouped advection
+  double *dgout[2] = {};                // results of its FP64 group (theta, pi) in the FP32 core
   long nblk_diag = 0;
   double *d_diag_out = nullptr;
 
@@ -2751,13 +2762,14 @@ struct Core : CoreBase {
     int rc;
 #define AF(ptr, nl) \
   if ((rc = alloc_field(&D.ptr, nl)) != MLC_OK) return rc;
-    AF(u, K) AF(v, K) AF(un, K) AF(vn, K) AF(w, K1) AF(t, K) AF(tetav, K);
+    AF(u, K) AF(v, K) AF(un, K) AF(vn, K) AF(w, K1);
     AF(qv, K) AF(qc, K) AF(qi, K) AF(qr, K) AF(qs, K);
     AF(tvirt, K) AF(p, K) AF(rho, K) AF(qsat, K) AF(ux, K) AF(vx, K) AF(ps, 1);
     AF(fmz, K) AF(fmzf, K1) AF(zeta, K) AF(hx, 1) AF(hy, 1) AF(mu, 1) AF(mv, 1) AF(mx, 1);
     AF(rmu, 1) AF(rmv, 1) AF(mx2, 1) AF(coru, 1) AF(corv, 1);
-    AF(s, K1) AF(deltaw, K1) AF(zdh, K) AF(zds, K) AF(zdf, K) AF(wx, K) AF(qwltot, K) AF(qwitot, K);
-    AF(tf, K) AF(sm, K);
+    AF(s, K1) AF(deltaw, K1) AF(zdh, K) AF(wx, K) AF(qwltot, K) AF(qwitot, K);
+    if (!segmented_solve() || p.divfilter) AF(zds, K);
+    if (p.divfilter) AF(zdf, K);
     for (int f = 0; f < MAXF; f++) {
       if ((rc = alloc_field(&gwz[f], K)) != MLC_OK) return rc;
       if ((rc = alloc_field(&gout[f], K)) != MLC_OK) return rc;
@@ -2766,10 +2778,26 @@ struct Core : CoreBase {
     if ((rc = alloc_d(&D.pai, g.plane * K)) != MLC_OK) return rc;
     if ((rc = alloc_d(&D.wz, g.plane * K)) != MLC_OK) return rc;
     if ((rc = alloc_d(&D.p0, g.plane * K)) != MLC_OK) return rc;
-    if ((rc = alloc_d(&D.pf, g.plane * K)) != MLC_OK) return rc;
-    if ((rc = alloc_d(&D.smd, g.plane * K)) != MLC_OK) return rc;
-    if ((rc = alloc_d(&D.thg, g.plane * K1)) != MLC_OK) return rc;
-    if ((rc = alloc_d(&D.thc, g.plane * K1)) != MLC_OK) return rc;
+    if ((rc = alloc_d(&D.t, g.plane * K)) != MLC_OK) return rc;
+    if ((rc = alloc_d(&D.tetav, g.plane * K)) != MLC_OK) return rc;
+    // scratch of the optional paths, only where they run
+    if (p.filterpai) {
+      if ((rc = alloc_d(&D.pf, g.plane * K)) != MLC_OK) return rc;
+      if ((rc = alloc_d(&D.smd, g.plane * K)) != MLC_OK) return rc;
+    }
+    if (p.fulleq && p.filtertheta) {
+      if ((rc = alloc_d(&D.tf, g.plane * K)) != MLC_OK) return rc;
+      if ((rc = alloc_d(&D.smt, g.plane * K)) != MLC_OK) return rc;
+    }
+    if constexpr (sizeof(T) != sizeof(double)) {
+      // results of the FP64 advection group (theta, pi)
+      for (int f = 0; f < 2; f++)
+        if ((rc = alloc_d(&dgout[f], g.plane * K)) != MLC_OK) return rc;
+    }
+    if (!segmented_solve()) {  // the one-lane column solve keeps its coefficients in HBM
+      if ((rc = alloc_d(&D.thg, g.plane * K1)) != MLC_OK) return rc;
+      if ((rc = alloc_d(&D.thc, g.plane * K1)) != MLC_OK) return rc;
+    }
     double *lv;
     if ((rc = alloc_d(&lv, 4L * K1)) != MLC_OK) return rc;
     D.gzitak = lv;
@@ -2809,8 +2837,8 @@ struct Core : CoreBase {
       case MLC_V: s.dev = D.v, s.nlev = K; break;
       case MLC_W: s.dev = D.w, s.nlev = K + 1; break;
       case MLC_PAI: s.dev = D.pai, s.nlev = K, s.is_double = true; break;
-      case MLC_T: s.dev = D.t, s.nlev = K; break;
-      case MLC_TETAV: s.dev = D.tetav, s.nlev = K; break;
+      case MLC_T: s.dev = D.t, s.nlev = K, s.is_double = true; break;
+      case MLC_TETAV: s.dev = D.tetav, s.nlev = K, s.is_double = true; break;
       case MLC_QV: s.dev = D.qv, s.nlev = K; break;
       case MLC_QC: s.dev = D.qc, s.nlev = K; break;
       case MLC_QI: s.dev = D.qi, s.nlev = K; break;
@@ -3007,10 +3035,10 @@ struct Core : CoreBase {
     const bool seg = segmented_solve();
     if (seg) {
       const int nbj = (g.nj + SEG_CW - 1) / SEG_CW;
-      launch(KK_SOUND_B, fbytes(13, 2), k_sound_bs<T, SEG_NS, SEG_L, SEG_CW>, dim3(nbj * g.ni),
+      launch(KK_SOUND_B, fbytes(11, 4), k_sound_bs<T, SEG_NS, SEG_L, SEG_CW>, dim3(nbj * g.ni),
              dim3(64, SEG_NS * SEG_CW / 64), D, P, dts, ns, split, nbj);
     } else {
-      launch(KK_SOUND_B, fbytes(13, 2), k_sound_b<T>, col_grid(), blk(), D, P, dts, ns, split);
+      launch(KK_SOUND_B, fbytes(11, 4), k_sound_b<T>, col_grid(), blk(), D, P, dts, ns, split);
     }
     if (split) {
       // divergence_filter (:423-438) then the Exner update
@@ -3023,13 +3051,13 @@ struct Core : CoreBase {
     // slower with them (79 against 63 ms per 2048 x 2048 x 60 step)
     const bool so = sizeof(T) == sizeof(float) && small_offsets();
     if (seg && so)
-      launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T, true, unsigned>, col_grid(), blk(), D, P, dts);
+      launch(KK_SOUND_C, fbytes(5, 2), k_sound_c<T, true, unsigned>, col_grid(), blk(), D, P, dts);
     else if (seg)
-      launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T, true, long>, col_grid(), blk(), D, P, dts);
+      launch(KK_SOUND_C, fbytes(5, 2), k_sound_c<T, true, long>, col_grid(), blk(), D, P, dts);
     else if (so)
-      launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T, false, unsigned>, col_grid(), blk(), D, P, dts);
+      launch(KK_SOUND_C, fbytes(5, 2), k_sound_c<T, false, unsigned>, col_grid(), blk(), D, P, dts);
     else
-      launch(KK_SOUND_C, fbytes(6, 1), k_sound_c<T, false, long>, col_grid(), blk(), D, P, dts);
+      launch(KK_SOUND_C, fbytes(5, 2), k_sound_c<T, false, long>, col_grid(), blk(), D, P, dts);
     std::swap(D.u, D.un);
     std::swap(D.v, D.vn);
   }
@@ -3065,8 +3093,14 @@ struct Core : CoreBase {
   // One group: NF fields of type Q (in *fld[f], scratch wzs[f], results into
   // outs[f], then swapped into place) and, with ND = 1, the FP64 rider
   // *dfld (scratch dwz, results into *dout, swapped likewise).
+  // Charged bytes: s, fmz and fmzf once per advection cycle (sfm_charged),
+  // and one read and one write of each advected field except the first
+  // nfree of the group (ux, vx, wx, which stand in for the u, v, w passes of
+  // k_adv_in and k_adv_out).
+  bool sfm_charged = false;
   template <typename Q, int NF, int ND>
-  void waf_group(Q **fld[], Q **outs, Q **wzs, double **dfld, double **dout, double *dwz, double dta) {
+  void waf_group(Q **fld[], Q **outs, Q **wzs, double **dfld, double **dout, double *dwz, double dta,
+                 int nfree = 0) {
     Par P = par();
     FieldSet<Q> F;
     for (int f = 0; f < MAXF; f++) F.in[f] = nullptr, F.wz[f] = nullptr, F.out[f] = nullptr;
@@ -3078,8 +3112,10 @@ struct Core : CoreBase {
     FieldD FD{nullptr, nullptr, nullptr};
     if (ND) FD = FieldD{*dfld, dwz, *dout};
     const double q = sizeof(Q) == sizeof(double) ? 1.0 : 0.0;
+    const int nc = NF - nfree;
     // vertical: s, fmz, fmzf + the fields -> wz (the rider on its own)
-    const double vb = fbytes(3, 0) + (q ? fbytes(0, NF) : fbytes(NF, 0));
+    const double vb = (sfm_charged ? 0.0 : fbytes(3, 0)) + (q ? fbytes(0, nc) : fbytes(nc, 0));
+    sfm_charged = true;
     if (small_offsets())
       launch(KK_WAF_V, vb, k_wafv<T, Q, NF, unsigned>, col_grid(), blk(), D, P, F, dta);
     else
@@ -3097,7 +3133,7 @@ struct Core : CoreBase {
     // horizontal: u, v, fmz + the fields' wz and starting values -> results
     const int ntj = (g.nj + HJ - 1) / HJ, nti = (g.ni + HR - 1) / HR;
     const long total = ((long)ntj * nti * g.nk + 3) / 4;  // blocks of four strips
-    const double hb = (q ? fbytes(0, NF) : fbytes(NF, 0)) + fbytes(0, ND);
+    const double hb = (q ? fbytes(0, nc) : fbytes(nc, 0)) + fbytes(0, ND);
     const dim3 hg((unsigned)round_up(total, 8));
     if (small_offsets()) {
       if (prm.rotllr)
@@ -3115,19 +3151,22 @@ struct Core : CoreBase {
   }
   // The n fields in groups of at most MAXF; the FP64 rider, if any, goes
   // with the last group.
+  // The first nfree fields are not charged (see waf_group).
   template <typename Q>
-  void waf_groups(Q **fld[], int n, Q **outs, Q **wzs, double **dfld, double **dout, double *dwz, double dta) {
+  void waf_groups(Q **fld[], int n, Q **outs, Q **wzs, double **dfld, double **dout, double *dwz, double dta,
+                  int nfree = 0) {
     for (int b = 0; b < n; b += MAXF) {
       const int m = n - b < MAXF ? n - b : MAXF;
       const bool rider = dfld && b + m == n;
+      const int nf = nfree - b < 0 ? 0 : (nfree - b > m ? m : nfree - b);
 #define MLC_WAF_GROUP(NF)                                                   \
   if constexpr (sizeof(Q) != sizeof(double)) {                              \
     if (rider) {                                                            \
-      waf_group<Q, NF, 1>(fld + b, outs, wzs, dfld, dout, dwz, dta);       \
+      waf_group<Q, NF, 1>(fld + b, outs, wzs, dfld, dout, dwz, dta, nf);   \
       break;                                                                \
     }                                                                       \
   }                                                                         \
-  waf_group<Q, NF, 0>(fld + b, outs, wzs, nullptr, nullptr, nullptr, dta);
+  waf_group<Q, NF, 0>(fld + b, outs, wzs, nullptr, nullptr, nullptr, dta, nf);
       switch (m) {
         case 1: MLC_WAF_GROUP(1) break;
         case 2: MLC_WAF_GROUP(2) break;
@@ -3148,25 +3187,29 @@ struct Core : CoreBase {
       T **qm[5] = {&D.qv, &D.qc, &D.qi, &D.qr, &D.qs};
       T **list[10];
       int n = 0;
-      list[n++] = &D.tetav;
-      if constexpr (sizeof(T) == sizeof(double)) list[n++] = reinterpret_cast<T **>(&D.pai);
       list[n++] = &D.ux;
       list[n++] = &D.vx;
       list[n++] = &D.wx;
+      if constexpr (sizeof(T) == sizeof(double)) {
+        list[n++] = reinterpret_cast<T **>(&D.tetav);
+        list[n++] = reinterpret_cast<T **>(&D.pai);
+      }
       list[n++] = &D.qv;
       for (int s = prm.iqfrst; s <= dims.nqx; s++) list[n++] = qm[s - 1];
+      sfm_charged = false;
+      waf_groups<T>(list, n, gout, gwz, nullptr, nullptr, nullptr, dta, 3);
       if constexpr (sizeof(T) != sizeof(double)) {
-        // the FP64 Exner function rides with the last group of FP32 fields
-        waf_groups<T>(list, n, gout, gwz, &D.pai, &D.smd, D.wz, dta);
-      } else {
-        waf_groups<T>(list, n, gout, gwz, nullptr, nullptr, nullptr, dta);
+        // theta and the Exner function: a group of their own in FP64
+        double **dl[2] = {&D.tetav, &D.pai};
+        double *dwz[2] = {D.p0, D.wz};
+        waf_group<double, 2, 0>(dl, dgout, dwz, nullptr, nullptr, nullptr, dta);
       }
       halo(D.ux, g.nk, 2);
       halo(D.vx, g.nk, 3);
       launch(KK_ADV_OUT, fbytes(3, 0), k_adv_out<T>, col_grid(), blk(), D);
       return;
     }
-    wafone<T>(D.tetav, dta);
+    wafone<double>(D.tetav, dta);
     wafone<double>(D.pai, dta);
     wafone<T>(D.ux, dta);
     wafone<T>(D.vx, dta);
@@ -3187,7 +3230,8 @@ struct Core : CoreBase {
     int nflp = 0, nin = 0, nout = 0, chunk = 0;
     LacRows rows{};
     long rtot = 0;
-    T *su = nullptr, *sv = nullptr, *sw = nullptr, *stet = nullptr;  // phase-start snapshot
+    T *su = nullptr, *sv = nullptr, *sw = nullptr;  // phase-start snapshot
+    double *stet = nullptr;
     double *spai = nullptr;
     T *ru = nullptr, *rv = nullptr, *rw = nullptr;  // long substep without increment
     double *rpai = nullptr;
@@ -3209,6 +3253,8 @@ struct Core : CoreBase {
     double *norms = nullptr;  // check accumulators
     int mode = MLC_LAC_OFF;
     long long learned_steps = 0;
+    int retrain_left = 0;  // collection steps left before an automatic retraining
+    int consecutive = 0;   // retrainings since the last passed check
     mlc_lac_stats st{};
     uint64_t rng = 0;
     uint64_t next_u64() {
@@ -3226,6 +3272,9 @@ struct Core : CoreBase {
     if (c.nhid < 16 || c.samples_per_phase < 1 || c.capacity < 1 || c.chunk_rows < 256 || c.chunk_rows % 256 ||
         c.check_every < 0 || !(c.check_tol >= 0.0) || (c.engine != 0 && c.engine != 1))
       return mlc_fail(MLC_EINVAL, "mlc_lac_create: bad configuration");
+    if (!(c.rel_tol >= 0.0) || c.retrain_steps < 0 || c.max_retrains < 0 ||
+        (c.retrain_steps > 0 && (c.retrain_iters < 1 || c.retrain_batch < 1 || !(c.retrain_lr > 0.0))))
+      return mlc_fail(MLC_EINVAL, "mlc_lac_create: bad drift-check or retraining settings");
     if (c.engine == 1 && c.nhid % 64)
       return mlc_fail(MLC_EINVAL, "mlc_lac_create: the fused engine needs nhid a multiple of 64");
     L.cfg = c;
@@ -3242,7 +3291,7 @@ struct Core : CoreBase {
   if ((rc = alloc_bytes((void **)&ptr, (size_t)(bytes))) != MLC_OK) return rc;
     const size_t fT = (size_t)g.plane * K * sizeof(T), fT1 = (size_t)g.plane * (K + 1) * sizeof(T);
     const size_t fD = (size_t)g.plane * K * sizeof(double);
-    AL(L.su, fT) AL(L.sv, fT) AL(L.sw, fT1) AL(L.stet, fT) AL(L.spai, fD);
+    AL(L.su, fT) AL(L.sv, fT) AL(L.sw, fT1) AL(L.stet, fD) AL(L.spai, fD);
     AL(L.ru, fT) AL(L.rv, fT) AL(L.rw, fT1) AL(L.rpai, fD);
     AL(L.cu, fT) AL(L.cv, fT) AL(L.cw, fT1) AL(L.cpai, fD);
     AL(L.Y, (size_t)L.rtot * L.nout * sizeof(float));
@@ -3424,7 +3473,7 @@ struct Core : CoreBase {
   // One acoustic phase with the long substep.  correct: add the network's
   // increment; check: also run the ordinary substeps from the same start,
   // compare, and continue from the ordinary result.
-  int sound_lac(double dts, bool correct, bool check, double *worst) {
+  int sound_lac(double dts, bool correct, bool check, double *worst, double *worst_rel) {
     if (!prm.fulleq) halo(D.tetav, g.nk, 0);
     const int nb = prm.nsound - 1;
     int rc;
@@ -3455,6 +3504,7 @@ struct Core : CoreBase {
         L.st.rel[f] = ref > 0.0 ? std::sqrt(ec / ref) : std::sqrt(ec);
         // a NaN ratio (a network that puts out NaN) stays the worst
         if (!std::isnan(*worst) && !(ratio <= *worst)) *worst = ratio;
+        if (!std::isnan(*worst_rel) && !(L.st.rel[f] <= *worst_rel)) *worst_rel = L.st.rel[f];
       }
       L.st.checks++;
     }
@@ -3586,9 +3636,12 @@ struct Core : CoreBase {
     }
     HIP_TRY(hipStreamSynchronize(stream));
     HIP_TRY(hipGetLastError());
-    if (iters > 0) L.st.last_loss = l;
+    if (iters > 0) {
+      L.st.last_loss = l;
+      L.st.fallback = 0;
+      L.retrain_left = 0;
+    }
     if (loss) *loss = l;
-    L.st.fallback = 0;
     return MLC_OK;
   }
 
@@ -3611,6 +3664,8 @@ struct Core : CoreBase {
     if (mode == MLC_LAC_ON && !L.normalised) return mlc_fail(MLC_ESTATE, "mlc_lac_mode: the corrector is not trained");
     L.mode = mode;
     L.learned_steps = 0;
+    L.retrain_left = 0;
+    L.consecutive = 0;
     return MLC_OK;
   }
 
@@ -3627,8 +3682,8 @@ struct Core : CoreBase {
     const long n2 = (long)g.nj * g.ni;
     hipLaunchKernelGGL(k_pack_interior<T>, dim3((g.nj + 63) / 64, (g.ni + 3) / 4, 1), dim3(256), 0, stream,
                        (const T *)D.ps, g, dst);
-    hipLaunchKernelGGL(k_pack_interior<T>, dim3((g.nj + 63) / 64, (g.ni + 3) / 4, g.nk), dim3(256), 0, stream,
-                       (const T *)D.t, g, dst + n2);
+    hipLaunchKernelGGL(k_pack_interior<double>, dim3((g.nj + 63) / 64, (g.ni + 3) / 4, g.nk), dim3(256), 0, stream,
+                       (const double *)D.t, g, dst + n2);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(done, stream));
     return MLC_OK;
@@ -3672,16 +3727,18 @@ struct Core : CoreBase {
     c.zita = d2 + 2 * n2;
     c.zitah = d2 + 2 * n2 + K + 1;
     // fields the case leaves at zero
-    T *zf[] = {D.w, D.tetav, D.qc, D.qi, D.qr, D.qs, D.tvirt, D.p, D.rho, D.qsat, D.ux, D.vx};
-    const int zl[] = {K + 1, K, K, K, K, K, K, K, K, K, K, K};
-    for (int f = 0; f < 12 && e == hipSuccess; f++) e = hipMemsetAsync(zf[f], 0, g.plane * zl[f] * sizeof(T), stream);
+    T *zf[] = {D.w, D.qc, D.qi, D.qr, D.qs, D.tvirt, D.p, D.rho, D.qsat, D.ux, D.vx};
+    const int zl[] = {K + 1, K, K, K, K, K, K, K, K, K, K};
+    for (int f = 0; f < 11 && e == hipSuccess; f++) e = hipMemsetAsync(zf[f], 0, g.plane * zl[f] * sizeof(T), stream);
+    if (e == hipSuccess) e = hipMemsetAsync(D.tetav, 0, g.plane * K * sizeof(double), stream);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(k_case_build<T>, col_grid(), blk(), 0, stream, D, c);
       const dim3 pg((unsigned)(((g.nj + 4) * (g.ni + 4) + 255) / 256), 1);
-      T *pf[] = {D.u, D.v, D.t, D.qv, D.zeta, D.fmz};
+      T *pf[] = {D.u, D.v, D.qv, D.zeta, D.fmz};
       for (T *f : pf) hipLaunchKernelGGL(k_periodic<T>, dim3(pg.x, K), dim3(256), 0, stream, f, g);
       hipLaunchKernelGGL(k_periodic<T>, dim3(pg.x, K + 1), dim3(256), 0, stream, D.fmzf, g);
       hipLaunchKernelGGL(k_periodic<double>, dim3(pg.x, K), dim3(256), 0, stream, D.pai, g);
+      hipLaunchKernelGGL(k_periodic<double>, dim3(pg.x, K), dim3(256), 0, stream, D.t, g);
       e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -3714,19 +3771,22 @@ struct Core : CoreBase {
     const double dtsound = dtstepa / (double)prm.nsound;
     // acoustic loop of this step
     const bool learned = L.exists && L.mode == MLC_LAC_ON && !L.st.fallback && !collect;
+    // after a failed check: ordinary steps that collect fresh training pairs
+    const bool recollect = L.exists && L.mode == MLC_LAC_ON && L.st.fallback && L.retrain_left > 0 && !collect;
     const bool reduced = L.exists && L.mode == MLC_LAC_REDUCED && !collect;
     if (learned && !L.normalised) return mlc_fail(MLC_ESTATE, "mlc_step: the corrector is not trained");
     const bool check = learned && L.cfg.check_every > 0 && L.learned_steps % L.cfg.check_every == 0;
     Par P = par();
-    launch(KK_PROLOGUE, fbytes(10, 1), k_prologue<T>, col_grid(), blk(), D, P, prm.filterpai ? 1 : 0,
+    launch(KK_PROLOGUE, fbytes(8, 3), k_prologue<T>, col_grid(), blk(), D, P, prm.filterpai ? 1 : 0,
            (prm.fulleq && prm.filtertheta) ? 1 : 0);
-    double worst = 0.0;
+    double worst = 0.0, worst_rel = 0.0;
     for (int n = 1; n <= prm.nadv; n++) {
       int rc = MLC_OK;
-      if (collect)
+      if (collect || recollect) {
         rc = sound_collect(dtsound);
-      else if (learned || reduced)
-        rc = sound_lac(dtsound, learned, check, &worst);
+        if (recollect) L.st.collect_phases++;
+      } else if (learned || reduced)
+        rc = sound_lac(dtsound, learned, check, &worst, &worst_rel);
       else {
         sound(dtsound);
         if (L.exists) L.st.plain_phases++;
@@ -3738,9 +3798,23 @@ struct Core : CoreBase {
       L.learned_steps++;
       if (check) {
         L.st.worst_ratio = worst;
-        if (!(worst <= L.cfg.check_tol)) L.st.fallback = 1;  // NaN included
+        L.st.worst_rel = worst_rel;
+        // NaN included: the comparisons are false
+        const bool ok = worst <= L.cfg.check_tol && (L.cfg.rel_tol <= 0.0 || worst_rel <= L.cfg.rel_tol);
+        if (ok) {
+          L.consecutive = 0;
+        } else {
+          L.st.fallback = 1;
+          L.st.failed_checks++;
+          if (L.cfg.retrain_steps > 0 && L.consecutive < L.cfg.max_retrains) {
+            L.retrain_left = L.cfg.retrain_steps;
+            L.count = 0;  // the training set restarts with the current flow
+            L.next = 0;
+          }
+        }
       }
     }
+    const bool retrain_now = recollect && --L.retrain_left == 0;
     if (prm.filterpai) {  // :333-341
       launch(KK_FILTER_BRACKET, fbytes(0, 3), k_sub<double, double>, pt_grid(), blk(), D.pai, (const double *)D.pf, g);
       halo(D.pai, g.nk, 0);
@@ -3750,15 +3824,26 @@ struct Core : CoreBase {
              (const double *)D.smd, (const double *)D.pf, g);
     }
     if (prm.fulleq && prm.filtertheta) {  // :343-353
-      launch(KK_FILTER_BRACKET, fbytes(3, 0), k_sub<T, T>, pt_grid(), blk(), D.tetav, (const T *)D.tf, g);
+      launch(KK_FILTER_BRACKET, fbytes(0, 3), k_sub<double, double>, pt_grid(), blk(), D.tetav,
+             (const double *)D.tf, g);
       halo(D.tetav, g.nk, 0);
-      launch(KK_SMOOTH5, fbytes(2, 0), k_smooth5<T>, pt_grid(), blk(), (const T *)D.tetav, D.sm, g,
+      launch(KK_SMOOTH5, fbytes(0, 2), k_smooth5<double>, pt_grid(), blk(), (const double *)D.tetav, D.smt, g,
              (const double *)nullptr, kNupaitq);
-      launch(KK_FILTER_BRACKET, fbytes(3, 0), k_addto<T, T>, pt_grid(), blk(), D.tetav, (const T *)D.sm,
-             (const T *)D.tf, g);
+      launch(KK_FILTER_BRACKET, fbytes(0, 3), k_addto<double, double>, pt_grid(), blk(), D.tetav,
+             (const double *)D.smt, (const double *)D.tf, g);
     }
-    launch(KK_EPILOGUE, fbytes(11, 1), k_epilogue<T>, col_grid(), blk(), D);
+    launch(KK_EPILOGUE, fbytes(9, 3), k_epilogue<T>, col_grid(), blk(), D);
     HIP_TRY(hipGetLastError());
+    if (retrain_now) {
+      // retrain on the fresh pairs, normalisation recomputed, then resume
+      // with a check at the next step
+      L.consecutive++;
+      L.st.retrains++;
+      L.normalised = false;
+      int rc = lac_train(L.cfg.retrain_iters, L.cfg.retrain_batch, L.cfg.retrain_lr, nullptr);
+      if (rc) return rc;
+      L.learned_steps = 0;
+    }
     if (lprint) {
       launch(KK_DIAG, fbytes(2, 0), k_diag_partial<T>, col_grid(), blk(), D);
       launch(KK_DIAG, 0.0, k_diag_final, dim3(1), dim3(256), D.diag, nblk_diag, d_diag_out);
@@ -3974,6 +4059,12 @@ extern "C" int mlc_lac_default_config(mlc_lac_config *c) {
   c->check_tol = 1.0;
   c->seed = 1;
   c->engine = 1;
+  c->rel_tol = 1e-3;
+  c->retrain_steps = 2;
+  c->retrain_iters = 1500;
+  c->retrain_batch = 1024;
+  c->retrain_lr = 3e-4;
+  c->max_retrains = 3;
   return MLC_OK;
 }
 

@@ -56,34 +56,50 @@ class MLP:
     def dims(self, layer: int) -> tuple[int, int]:
         return ([self.nin, self.nhid, self.nhid][layer], [self.nhid, self.nhid, self.nout][layer])
 
+    def _layer(self, layer: int) -> tuple[int, int]:
+        if layer not in (0, 1, 2):
+            raise ValueError(f"layer {layer}: the network has layers 0, 1, 2")
+        return self.dims(layer)
+
+    def _rows(self, x: np.ndarray, what: str) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        if x.ndim != 2 or x.shape[1] != self.nin:
+            raise ValueError(f"{what}: inputs of shape {x.shape}, expected (rows, {self.nin})")
+        return x
+
     def get_weights(self, layer: int):
-        i, o = self.dims(layer)
+        i, o = self._layer(layer)
         w = np.zeros((i, o), np.float32)
         b = np.zeros(o, np.float32)
         check("mlc_mlp_weights", load().mlc_mlp_weights(self._h, layer, w.ctypes.data, b.ctypes.data, 0))
         return w, b
 
     def set_weights(self, layer: int, w: np.ndarray, b: np.ndarray) -> None:
+        i, o = self._layer(layer)
         w = np.ascontiguousarray(w, np.float32)
         b = np.ascontiguousarray(b, np.float32)
+        if w.shape != (i, o) or b.shape != (o,):
+            raise ValueError(f"layer {layer}: weights {w.shape} and biases {b.shape}, expected ({i}, {o}) and ({o},)")
         check("mlc_mlp_weights", load().mlc_mlp_weights(self._h, layer, w.ctypes.data, b.ctypes.data, 1))
 
     def forward(self, x: np.ndarray) -> np.ndarray:
-        x = np.ascontiguousarray(x, np.float32)
+        x = self._rows(x, "forward")
         y = np.zeros((x.shape[0], self.nout), np.float32)
         check("mlc_mlp_forward", load().mlc_mlp_forward(self._h, x.ctypes.data, x.shape[0], y.ctypes.data))
         return y
 
     def grad(self, x: np.ndarray, t: np.ndarray) -> float:
-        x = np.ascontiguousarray(x, np.float32)
+        x = self._rows(x, "grad")
         t = np.ascontiguousarray(t, np.float32)
+        if t.shape != (x.shape[0], self.nout):
+            raise ValueError(f"grad: targets of shape {t.shape}, expected ({x.shape[0]}, {self.nout})")
         loss = C.c_double()
         check("mlc_mlp_grad", load().mlc_mlp_grad(self._h, x.ctypes.data, t.ctypes.data, x.shape[0],
                                                   C.byref(loss)))
         return loss.value
 
     def gradients(self, layer: int):
-        i, o = self.dims(layer)
+        i, o = self._layer(layer)
         gw = np.zeros((i, o), np.float32)
         gb = np.zeros(o, np.float32)
         check("mlc_mlp_gradients", load().mlc_mlp_gradients(self._h, layer, gw.ctypes.data, gb.ctypes.data))
@@ -140,10 +156,16 @@ class Corrector:
         check("mlc_lac_get_stats", load().mlc_lac_get_stats(self.model._h, C.byref(s)))
         return {"samples": s.samples, "learned_phases": s.learned_phases, "plain_phases": s.plain_phases,
                 "checks": s.checks, "fallback": bool(s.fallback), "last_loss": s.last_loss,
-                "ratio": list(s.ratio), "rel": list(s.rel), "worst_ratio": s.worst_ratio}
+                "ratio": list(s.ratio), "rel": list(s.rel), "worst_ratio": s.worst_ratio,
+                "worst_rel": s.worst_rel, "retrains": s.retrains, "collect_phases": s.collect_phases,
+                "failed_checks": s.failed_checks}
 
-    def dataset(self, first: int, n: int, kz: int):
-        """Rows of the training set: (X raw inputs, T targets, sampled columns)."""
+    def dataset(self, first: int, n: int):
+        """Rows of the training set: (X raw inputs, T targets, sampled columns);
+        the widths follow the handle's level count."""
+        kz = self.model.kz
+        if first < 0 or n < 0:
+            raise ValueError("dataset: negative row range")
         X = np.zeros((n, nin_for(kz)), np.float32)
         T = np.zeros((n, nout_for(kz)), np.float32)
         cols = np.zeros(n, np.int32)
@@ -151,10 +173,12 @@ class Corrector:
                                                         T.ctypes.data, cols.ctypes.data))
         return X, T, cols
 
-    def network(self, kz: int) -> MLP:
+    def network(self) -> MLP:
+        """The handle's network (dimensions from the handle's level count)."""
         ptr = load().mlc_lac_network(self.model._h)
         if not ptr:
             raise RuntimeError("the handle has no corrector network")
+        kz = self.model.kz
         return MLP(nin_for(kz), self.config.nhid, nout_for(kz), _borrowed=ptr)
 
 

@@ -62,6 +62,21 @@ typedef struct mlc_lac_config {
   unsigned long long seed;
   int engine;            /* inference: 1 fused MFMA kernels reading the fields,
                           * 0 materialised features + the generic GEMM       */
+  /* the error budget of a check: fall back also when a field's relative L2
+   * error with the increment (rel below) exceeds rel_tol or is NaN; 0: no
+   * budget, only check_tol                                                 */
+  double rel_tol;
+  /* after a failed check: retrain_steps ordinary steps that collect fresh
+   * pairs (the training set restarts), then retrain_iters Adam steps of
+   * retrain_batch rows at retrain_lr with the normalisation recomputed,
+   * then the corrector resumes with a check; at most max_retrains times in
+   * a row without a passed check, after which it stays off until trained
+   * again.  retrain_steps = 0: no retraining (stay off)                    */
+  int retrain_steps;
+  int retrain_iters;
+  int retrain_batch;
+  double retrain_lr;
+  int max_retrains;
 } mlc_lac_config;
 
 typedef struct mlc_lac_stats {
@@ -77,6 +92,10 @@ typedef struct mlc_lac_stats {
   double ratio[4];
   double rel[4];
   double worst_ratio;       /* largest ratio over the phases of the last check */
+  double worst_rel;         /* largest rel over the phases of the last check   */
+  long long retrains;       /* retrainings after failed checks                 */
+  long long collect_phases; /* acoustic phases run to collect retraining pairs */
+  long long failed_checks;  /* checks that failed                              */
 } mlc_lac_stats;
 
 int mlc_lac_default_config(mlc_lac_config *c);
@@ -87,8 +106,9 @@ int mlc_lac_create(mlc_handle *h, const mlc_lac_config *c);
  * from every acoustic phase to the training set. */
 int mlc_lac_collect(mlc_handle *h, int nsteps);
 /* iters Adam steps on random batches of the training set; the first call
- * fixes the input and target normalisation.  *loss (may be NULL) receives
- * the last batch's loss.  Clears the fallback flag. */
+ * fixes the input and target normalisation (a retraining after a failed
+ * check recomputes it).  *loss (may be NULL) receives the last batch's
+ * loss.  With iters > 0 it clears the fallback flag. */
 int mlc_lac_train(mlc_handle *h, int iters, int batch, double lr, double *loss);
 /* Acoustic loop used by mlc_step: MLC_LAC_OFF, MLC_LAC_ON, MLC_LAC_REDUCED. */
 int mlc_lac_mode(mlc_handle *h, int mode);

@@ -62,7 +62,7 @@ def test_training_pairs_match_oracle():
     print(f"case 40x36x16 fp64, collect 1 step, {spp} samples per phase", flush=True)
     corr.collect(1)
     assert corr.stats()["samples"] == 3 * spp
-    X, T, cols = corr.dataset(0, spp, case.kz)  # the first acoustic phase
+    X, T, cols = corr.dataset(0, spp)  # the first acoustic phase
     m.close()
 
     kz, jx, iy = case.kz, case.jx, case.iy
@@ -140,17 +140,23 @@ def _train(corr, iters):
     return losses
 
 
-@pytest.mark.parametrize("precision", [dm.MLC_FP32, dm.MLC_FP64])
-def test_learned_increment_beats_long_substep(precision):
-    case = small_config2(64, 64, 16)
+@pytest.mark.parametrize("precision,kz,nhid", [(dm.MLC_FP32, 16, 256), (dm.MLC_FP64, 16, 256),
+                                               (dm.MLC_FP32, 40, 512)],
+                         ids=["fp32-kz16", "fp64-kz16", "fp32-kz40-corrector-shape"])
+def test_learned_increment_beats_long_substep(precision, kz, nhid):
+    """The last case is the corrector of the 512 x 512 x 40 training set:
+    2160 inputs, two hidden layers of 512, 160 outputs."""
+    case = small_config2(64, 64, kz)
     learned, _, _ = _model(case, precision=precision)
     ordinary, _, _ = _model(case, precision=precision)
     reduced, _, _ = _model(case, precision=precision)
-    cfg = dict(nhid=256, samples_per_phase=2048, capacity=65536, chunk_rows=4096, check_every=0, seed=3)
+    cfg = dict(nhid=nhid, samples_per_phase=2048, capacity=65536, chunk_rows=4096, check_every=0, seed=3)
     corr = Corrector(learned, **cfg)
     rcorr = Corrector(reduced, **cfg)
+    if kz == 40:
+        assert corr.network().nin == 2160 and corr.network().nout == 160
     nwarm = 8
-    print(f"case 64x64x16 precision={precision}: collect {nwarm} steps", flush=True)
+    print(f"case 64x64x{kz} precision={precision} nhid={nhid}: collect {nwarm} steps", flush=True)
     corr.collect(nwarm)
     for n in range(1, nwarm + 1):
         ordinary.moloch(n)
@@ -179,7 +185,7 @@ def test_learned_increment_beats_long_substep(precision):
 
 
 @pytest.mark.parametrize("shape,nhid", [((64, 64, 16), 128), ((70, 45, 20), 128), ((64, 64, 16), 512),
-                                        ((64, 48, 60), 512), ((64, 64, 9), 512)])
+                                        ((64, 64, 40), 512), ((64, 48, 60), 512), ((64, 64, 9), 512)])
 def test_fused_engine_matches_generic(shape, nhid):
     """The fused kernels (inputs gathered from the fields inside the MFMA
     kernel, packed weights; at 512 hidden units layers 2 and 3 in one kernel,
@@ -196,7 +202,7 @@ def test_fused_engine_matches_generic(shape, nhid):
         print(f"case {jx}x{iy}x{kz} nhid={nhid} fp32 engine={engine}", flush=True)
         corr.collect(2)
         corr.train(200, 256, 1e-3)
-        w = [corr.network(kz).get_weights(l) for l in range(3)]
+        w = [corr.network().get_weights(l) for l in range(3)]
         corr.mode("on")
         m.moloch(3)
         runs[engine] = (w, _fields(m), corr.stats())
@@ -217,7 +223,7 @@ def test_failed_check_falls_back_bitwise():
     a, _, _ = _model(case, precision=dm.MLC_FP32)
     b, _, _ = _model(case, precision=dm.MLC_FP32)
     corr = Corrector(a, nhid=64, samples_per_phase=512, capacity=4096, chunk_rows=1024, check_every=1,
-                     check_tol=0.0)
+                     check_tol=0.0, retrain_steps=0)
     print("case 48x40x16 fp32: check tolerance 0 forces the fallback", flush=True)
     corr.collect(2)
     for n in (1, 2):
@@ -232,7 +238,9 @@ def test_failed_check_falls_back_bitwise():
     print(st)
     assert st["fallback"] and st["checks"] == 3 and st["learned_phases"] == 0
     assert all(r > 0 for r in st["ratio"])
-    # retraining clears the fallback
+    # training with no iterations leaves the fallback; retraining clears it
+    corr.train(0, 128, 1e-4)
+    assert corr.stats()["fallback"]
     corr.train(1, 128, 1e-4)
     assert not corr.stats()["fallback"]
     a.close()
@@ -247,13 +255,13 @@ def test_nan_network_falls_back_bitwise():
     a, _, _ = _model(case, precision=dm.MLC_FP32)
     b, _, _ = _model(case, precision=dm.MLC_FP32)
     corr = Corrector(a, nhid=64, samples_per_phase=512, capacity=4096, chunk_rows=1024, check_every=1,
-                     check_tol=1.0)
+                     check_tol=1.0, retrain_steps=0)
     print("case 48x40x16 fp32: NaN network output", flush=True)
     corr.collect(2)
     for n in (1, 2):
         b.moloch(n)
     corr.train(20, 128, 1e-3)
-    net = corr.network(case.kz)
+    net = corr.network()
     w, bias = net.get_weights(2)
     bias[:] = np.nan
     net.set_weights(2, w, bias)
@@ -275,7 +283,7 @@ def test_checks_pass_with_loose_tolerance():
     case = small_config2(48, 40, 16)
     a, _, _ = _model(case, precision=dm.MLC_FP32)
     corr = Corrector(a, nhid=64, samples_per_phase=512, capacity=4096, chunk_rows=1024, check_every=2,
-                     check_tol=1e30)
+                     check_tol=1e30, rel_tol=0.0)
     print("case 48x40x16 fp32: checks every 2 steps, loose tolerance", flush=True)
     corr.collect(2)
     corr.train(50, 128, 1e-3)
@@ -287,7 +295,7 @@ def test_checks_pass_with_loose_tolerance():
     # learned steps 0, 2, 4 are checked (3 phases each), the others use the increment
     assert st["checks"] == 9 and st["learned_phases"] == 6 and not st["fallback"]
     assert all(np.isfinite(st["rel"]))
-    net = corr.network(case.kz)
+    net = corr.network()
     w, b = net.get_weights(0)
     assert w.shape == (nin_for(case.kz), 64) and np.isfinite(w).all()
     a.close()
@@ -311,3 +319,68 @@ def test_corrector_errors():
     with pytest.raises(dm.MolochError):
         Corrector(b, nhid=64, samples_per_phase=16, capacity=64, chunk_rows=256)  # second corrector
     b.close()
+
+
+def test_relative_error_budget_gates_the_check():
+    """A trained network beats the uncorrected long substep (error ratios
+    below 1, which check_tol = 1 accepts) but not a tight budget on the
+    relative error: with rel_tol below its error the check fails and the run
+    goes on with the ordinary loop; with a loose rel_tol it passes."""
+    case = small_config2(64, 64, 16)
+    out = {}
+    for tag, rel_tol in (("tight", 1e-9), ("loose", 1e3)):
+        m, _, _ = _model(case, precision=dm.MLC_FP32)
+        corr = Corrector(m, nhid=256, samples_per_phase=2048, capacity=65536, chunk_rows=4096, check_every=1,
+                         check_tol=1.0, rel_tol=rel_tol, retrain_steps=0, seed=3)
+        print(f"case 64x64x16 fp32: rel_tol {rel_tol}", flush=True)
+        corr.collect(8)
+        _train(corr, 1000)
+        corr.mode("on")
+        m.moloch(9)
+        out[tag] = corr.stats()
+        print(tag, out[tag])
+        m.close()
+    for tag in out:
+        assert out[tag]["worst_ratio"] < 1.0, tag  # better than no increment
+        assert 0.0 < out[tag]["worst_rel"] < 1.0
+    assert out["tight"]["fallback"] and out["tight"]["failed_checks"] == 1
+    assert not out["loose"]["fallback"] and out["loose"]["failed_checks"] == 0
+
+
+def test_failed_check_recollects_retrains_and_resumes():
+    """After a failed check the handle collects fresh pairs for
+    retrain_steps ordinary steps, retrains and resumes with a check; after
+    max_retrains retrainings without a passed check it stays off.  With a
+    budget no network meets, every step is a failed check (which continues
+    from the ordinary substeps), a collection step or an ordinary step, so
+    the trajectory stays bit-identical to the ordinary loop."""
+    case = small_config2(48, 40, 16)
+    a, _, _ = _model(case, precision=dm.MLC_FP32)
+    b, _, _ = _model(case, precision=dm.MLC_FP32)
+    corr = Corrector(a, nhid=64, samples_per_phase=512, capacity=4096, chunk_rows=1024, check_every=50,
+                     check_tol=1e30, rel_tol=1e-12, retrain_steps=2, retrain_iters=20, retrain_batch=128,
+                     retrain_lr=1e-3, max_retrains=2)
+    print("case 48x40x16 fp32: unreachable budget, 2 retrainings", flush=True)
+    corr.collect(2)
+    for n in (1, 2):
+        b.moloch(n)
+    corr.train(20, 128, 1e-3)
+    corr.mode("on")
+    for n in range(3, 11):
+        a.moloch(n)
+        b.moloch(n)
+    _same_interior(_fields(a), _fields(b))
+    st = corr.stats()
+    print(st)
+    # steps 3, 6, 9 checked (the first learned step after each training),
+    # 4-5 and 7-8 collect, 10 ordinary
+    assert st["failed_checks"] == 3 and st["retrains"] == 2 and st["fallback"]
+    assert st["collect_phases"] == 4 * case_nadv(a) and st["learned_phases"] == 0
+    assert st["checks"] == 3 * case_nadv(a)
+    assert st["samples"] == 2 * case_nadv(a) * 512  # the set restarted with the last retraining
+    a.close()
+    b.close()
+
+
+def case_nadv(m):
+    return m.params.nadv

@@ -56,9 +56,14 @@ def _ref_grads(x, t, W, b):
     return loss, y, [(g1, gb1), (g2, gb2), (g3, gb3)]
 
 
-def test_mlp_forward_and_gradients_match_reference():
+@pytest.mark.parametrize("nin,nhid,nout,rows", [(108, 64, 16, 200), (2160, 512, 160, 512)],
+                         ids=["small", "corrector-kz40"])
+def test_mlp_forward_and_gradients_match_reference(nin, nhid, nout, rows):
+    """The engine against float64 with the same BF16 rounding points.  What
+    remains is FP32 accumulation, and the rare activation whose BF16 rounding
+    it flips to the neighbouring value (2^-8 relative); the second case is the
+    corrector's network at 512 x 512 x 40 (2160 -> 512 -> 512 -> 160)."""
     rng = np.random.default_rng(3)
-    nin, nhid, nout, rows = 108, 64, 16, 200
     net = MLP(nin, nhid, nout, seed=5)
     W, B = [], []
     for l in range(3):
@@ -71,16 +76,55 @@ def test_mlp_forward_and_gradients_match_reference():
     t = rng.normal(size=(rows, nout)).astype(np.float32)
     loss_ref, y_ref, grads_ref = _ref_grads(x, t, W, B)
     y = net.forward(x)
-    assert np.abs(y - y_ref).max() / np.abs(y_ref).max() < 2e-2
+    ey = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
+    eym = np.abs(y - y_ref).max() / np.abs(y_ref).max()
+    print(f"forward: rel L2 {ey:.2e} max {eym:.2e}")
+    assert ey < 1e-3 and eym < 4e-3
     loss = net.grad(x, t)
-    assert abs(loss - loss_ref) / loss_ref < 1e-2
+    print(f"loss rel err {abs(loss - loss_ref) / loss_ref:.2e}")
+    assert abs(loss - loss_ref) / loss_ref < 1e-4
     for l in range(3):
         gw, gb = net.gradients(l)
         rw, rb = grads_ref[l]
         ew = np.linalg.norm(gw - rw) / np.linalg.norm(rw)
         eb = np.linalg.norm(gb - rb) / np.linalg.norm(rb)
         print(f"layer {l}: rel err W {ew:.2e} b {eb:.2e}")
-        assert ew < 3e-2 and eb < 3e-2, l
+        assert ew < 2e-3 and eb < 2e-3, l
+    net.close()
+
+
+def test_adam_steps_match_numpy():
+    """Two Adam steps (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected)
+    against float64 numpy from the engine's own gradients: the moments, the
+    bias corrections and the update.  At the first step the bias-corrected
+    update is lr * sign(g); without the corrections it would be 3.16 lr."""
+    rng = np.random.default_rng(4)
+    nin, nhid, nout, rows = 64, 64, 32, 128
+    net = MLP(nin, nhid, nout, seed=8)
+    x = rng.normal(size=(rows, nin)).astype(np.float32)
+    t = rng.normal(size=(rows, nout)).astype(np.float32)
+    lr = 1e-3
+    w = [np.concatenate([a.ravel() for a in net.get_weights(l)]).astype(np.float64) for l in range(3)]
+    m = [np.zeros_like(a) for a in w]
+    v = [np.zeros_like(a) for a in w]
+    for step in (1, 2):
+        net.grad(x, t)
+        g = [np.concatenate([a.ravel() for a in net.gradients(l)]).astype(np.float64) for l in range(3)]
+        net.adam(lr)
+        for l in range(3):
+            m[l] = 0.9 * m[l] + 0.1 * g[l]
+            v[l] = 0.999 * v[l] + 0.001 * g[l] ** 2
+            upd = lr * (m[l] / (1 - 0.9 ** step)) / (np.sqrt(v[l] / (1 - 0.999 ** step)) + 1e-8)
+            got = np.concatenate([a.ravel() for a in net.get_weights(l)]).astype(np.float64)
+            dw = w[l] - got
+            live = np.abs(g[l]) > 1e-6
+            err = np.abs(dw - upd)[live].max() / lr
+            print(f"step {step} layer {l}: max |update error| / lr {err:.2e}")
+            assert err < 1e-3, (step, l)
+            if step == 1:
+                big = np.abs(g[l]) > 1e-4  # eps = 1e-8 negligible against |g|
+                np.testing.assert_allclose(dw[big], lr * np.sign(g[l][big]), rtol=1e-3)
+            w[l] = got
     net.close()
 
 
