@@ -123,6 +123,7 @@ EXPORTS = {
     "mlc_mlp_weights": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int]),
     "mlc_mlp_gradients": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "mlc_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "mlc_mlp_forward_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     "mlc_mlp_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     "mlc_mlp_adam": (C.c_int, [C.c_void_p, C.c_double]),
     "mlc_gemm_bf16_host": (C.c_int, [C.c_void_p, C.c_long, C.c_long, C.c_long, C.c_void_p, C.c_long, C.c_long,

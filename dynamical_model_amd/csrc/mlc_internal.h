@@ -25,6 +25,7 @@ int case_host_part(const mlc_case *c, CaseHost &o);
 
 struct mlc_mlp {
   int nin, nhid, nout;
+  int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = true;  // false when the stream belongs to a model handle
   // fp32 master weights (row-major [in][out]) and biases, bf16 copies

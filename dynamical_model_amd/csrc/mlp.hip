@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "mlc_internal.h"
@@ -271,23 +272,29 @@ int mlp_reserve_rows(mlc_mlp *p, int rows) {
     if (e_ != hipSuccess) return mlc_fail(MLC_EHIP, "%s: %s", #x, hipGetErrorString(e_));    \
   } while (0)
 
+// The three layers of rows X[rows x nin] (bf16, row stride ldx) through the
+// hidden activations H1, H2 (bf16, rows x nhid) into Y (fp32, row stride ldy).
+static int mlp_layers(mlc_mlp *p, const __bf16 *X, long ldx, int rows, __bf16 *H1, __bf16 *H2, float *Y, long ldy) {
+  int rc;
+  GemmArgs a{};
+  a.A = X, a.sam = ldx, a.sak = 1, a.B = p->Wb[0], a.sbk = p->nhid, a.sbn = 1;
+  a.M = rows, a.N = p->nhid, a.K = p->nin, a.bias = p->bvec[0], a.relu = 1, a.Cb = H1, a.ldc = p->nhid;
+  if ((rc = gemm(p->stream, a))) return rc;
+  a = GemmArgs{};
+  a.A = H1, a.sam = p->nhid, a.sak = 1, a.B = p->Wb[1], a.sbk = p->nhid, a.sbn = 1;
+  a.M = rows, a.N = p->nhid, a.K = p->nhid, a.bias = p->bvec[1], a.relu = 1, a.Cb = H2, a.ldc = p->nhid;
+  if ((rc = gemm(p->stream, a))) return rc;
+  a = GemmArgs{};
+  a.A = H2, a.sam = p->nhid, a.sak = 1, a.B = p->Wb[2], a.sbk = p->nout, a.sbn = 1;
+  a.M = rows, a.N = p->nout, a.K = p->nhid, a.bias = p->bvec[2], a.relu = 0, a.Cf = Y, a.ldc = ldy;
+  return gemm(p->stream, a);
+}
+
 // Forward of rows X[rows x nin] (bf16, row stride ldx): H1, H2, Y (fp32)
 int mlp_forward_dev(mlc_mlp *p, const __bf16 *X, long ldx, int rows, float *Yout, long ldy) {
   int rc = mlp_reserve(p, rows);
   if (rc) return rc;
-  GemmArgs a{};
-  a.A = X, a.sam = ldx, a.sak = 1, a.B = p->Wb[0], a.sbk = p->nhid, a.sbn = 1;
-  a.M = rows, a.N = p->nhid, a.K = p->nin, a.bias = p->bvec[0], a.relu = 1, a.Cb = p->H1, a.ldc = p->nhid;
-  if ((rc = gemm(p->stream, a))) return rc;
-  a = GemmArgs{};
-  a.A = p->H1, a.sam = p->nhid, a.sak = 1, a.B = p->Wb[1], a.sbk = p->nhid, a.sbn = 1;
-  a.M = rows, a.N = p->nhid, a.K = p->nhid, a.bias = p->bvec[1], a.relu = 1, a.Cb = p->H2, a.ldc = p->nhid;
-  if ((rc = gemm(p->stream, a))) return rc;
-  a = GemmArgs{};
-  a.A = p->H2, a.sam = p->nhid, a.sak = 1, a.B = p->Wb[2], a.sbk = p->nout, a.sbn = 1;
-  a.M = rows, a.N = p->nout, a.K = p->nhid, a.bias = p->bvec[2], a.relu = 0, a.Cf = Yout ? Yout : p->Y;
-  a.ldc = Yout ? ldy : p->nout;
-  return gemm(p->stream, a);
+  return mlp_layers(p, X, ldx, rows, p->H1, p->H2, Yout ? Yout : p->Y, Yout ? ldy : p->nout);
 }
 
 // Loss and gradients of one batch: X (bf16) and targets T (fp32, rows x nout).
@@ -386,12 +393,41 @@ static int mlp_sync_bf16(mlc_mlp *p) {
   return MLC_OK;
 }
 
+// ---------------------------------------- scratch of the allocation-free forward
+// mlc_mlp_forward_noalloc stages the bf16 inputs and the two hidden
+// activations of one call here instead of in memory it allocates.  Each
+// device the library runs on holds its own copy for the life of the process;
+// host code reaches it only through the address hipGetSymbolAddress gives
+// on that device (the array's name is a host-side stand-in, not a device
+// address).
+constexpr size_t kFwdScratchBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kFwdScratchBytes];
+static void *g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+// Look up this device's copy of the scratch once (device current).
+static int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return mlc_fail(MLC_EINVAL, "device %d out of range", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return MLC_OK;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return mlc_fail(MLC_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+  g_scratch[device] = ptr;
+  return MLC_OK;
+}
+
 int mlp_new(int nin, int nhid, int nout, unsigned long long seed, int device, hipStream_t stream, mlc_mlp **out) {
   if (!out || nin <= 0 || nhid <= 0 || nout <= 0) return mlc_fail(MLC_EINVAL, "mlc_mlp_create: bad arguments");
   *out = nullptr;
   MLP_TRY(hipSetDevice(device));
+  int src = scratch_setup(device);
+  if (src) return src;
   mlc_mlp *p = new mlc_mlp();
   p->nin = nin, p->nhid = nhid, p->nout = nout;
+  p->device = device;
   if (stream) {
     p->stream = stream;
     p->own_stream = false;
@@ -569,4 +605,36 @@ extern "C" int mlc_gemm_bf16_host(const float *A, long sam, long sak, long asize
   for (void *q : tmp.allocs) (void)hipFree(q);
   (void)hipStreamDestroy(tmp.stream);
   return rc;
+}
+
+// Forward of device arrays without allocating: X [rows x nin] fp32 and
+// Y [rows x nout] fp32 in device memory; the bf16 inputs and the hidden
+// activations go to the device's scratch.  The same kernels, in the same
+// order, as mlc_mlp_forward, so the results are the same bits.
+extern "C" int mlc_mlp_forward_noalloc(mlc_mlp *p, const float *X, int rows, float *Y) {
+  if (!p || !X || !Y || rows <= 0) return mlc_fail(MLC_EINVAL, "mlc_mlp_forward_noalloc: bad arguments");
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t xb = up((size_t)rows * p->nin * sizeof(__bf16));
+  const size_t hb = up((size_t)rows * p->nhid * sizeof(__bf16));
+  const size_t need = xb + 2 * hb;
+  if (need > kFwdScratchBytes) {
+    const size_t per_row = (size_t)p->nin * 2 + 2 * (size_t)p->nhid * 2;
+    return mlc_fail(MLC_EINVAL,
+                    "mlc_mlp_forward_noalloc: %d rows need %zu bytes of scratch, more than its %zu (at most about "
+                    "%zu rows of this network)",
+                    rows, need, kFwdScratchBytes, kFwdScratchBytes / per_row);
+  }
+  void *base = (p->device >= 0 && p->device < kMaxDevices) ? g_scratch[p->device] : nullptr;
+  if (!base) return mlc_fail(MLC_ESTATE, "mlc_mlp_forward_noalloc: no scratch on device %d", p->device);
+  unsigned char *sc = static_cast<unsigned char *>(base);
+  __bf16 *Xb = reinterpret_cast<__bf16 *>(sc);
+  __bf16 *H1 = reinterpret_cast<__bf16 *>(sc + xb);
+  __bf16 *H2 = reinterpret_cast<__bf16 *>(sc + xb + hb);
+  MLP_TRY(hipSetDevice(p->device));
+  const long n = (long)rows * p->nin;
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream, X, Xb, n);
+  int rc = mlp_layers(p, Xb, p->nin, rows, H1, H2, Y, p->nout);
+  if (rc) return rc;
+  MLP_TRY(hipStreamSynchronize(p->stream));
+  return MLC_OK;
 }

@@ -88,6 +88,12 @@ class MLP:
         check("mlc_mlp_forward", load().mlc_mlp_forward(self._h, x.ctypes.data, x.shape[0], y.ctypes.data))
         return y
 
+    def forward_noalloc(self, x_dev: int, rows: int, y_dev: int) -> None:
+        """Y = network(X) for device arrays (addresses of fp32 [rows x nin] and
+        [rows x nout]) without any device allocation by the library."""
+        check("mlc_mlp_forward_noalloc", load().mlc_mlp_forward_noalloc(self._h, C.c_void_p(x_dev), int(rows),
+                                                                        C.c_void_p(y_dev)))
+
     def grad(self, x: np.ndarray, t: np.ndarray) -> float:
         x = self._rows(x, "grad")
         t = np.ascontiguousarray(t, np.float32)

@@ -36,6 +36,16 @@ int mlc_mlp_weights(mlc_mlp *p, int layer, float *w, float *b, int set);
 int mlc_mlp_gradients(mlc_mlp *p, int layer, float *gw, float *gb);
 /* Y [rows x nout] = network(X [rows x nin]); host arrays, X rounded to BF16. */
 int mlc_mlp_forward(mlc_mlp *p, const float *X, int rows, float *Y);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Y [rows x nout] = network(X [rows x nin]) like
+ * mlc_mlp_forward, the same kernels and bits, but X and Y are DEVICE arrays
+ * (fp32) and the bf16 inputs and hidden activations live in a fixed 64 MiB
+ * scratch of the library on the network's device, so the call allocates
+ * nothing; rows beyond what fits (rows * (2 nin + 4 nhid) bytes) are refused
+ * with MLC_EINVAL.  Calls that overlap on one device share that scratch and
+ * race; the call returns after the network's stream has finished. */
+int mlc_mlp_forward_noalloc(mlc_mlp *p, const float *X, int rows, float *Y);
 /* Mean-square-error loss against T [rows x nout] and its gradients. */
 int mlc_mlp_grad(mlc_mlp *p, const float *X, const float *T, int rows, double *loss);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the last gradients. */
