@@ -4,12 +4,17 @@
 Metric: column-steps/s per GPU -- one (j,i) column advanced by one
 moloch(istep, lprint) call (dt = 30 s: 3 advection cycles x 5 acoustic
 substeps, nompi/mod_moloch.F90:272-421) -- for the plain FP64 core, the FP32
-core (FP64 Exner function, hydrostatic residuals and column solve) and the
-FP32 core with the learned acoustic corrector (one long acoustic substep plus
-a BF16 MFMA network increment, then the last ordinary substep; the network is
-trained on the GPU from ordinary steps of the same run before timing), with
-each variant's relative L2 and max-abs error against the FP64 core after the
-same steps (dynamical_model_amd.errgrowth follows them over long runs).
+core (FP64 Exner function, theta, temperature, hydrostatic residuals and
+column solve) and the FP32 core with the learned acoustic corrector (one long
+acoustic substep plus a BF16 MFMA network increment, then the last ordinary
+substep; the network is trained on the GPU from ordinary steps of the same
+run before timing, and a check every 50 steps against the ordinary substeps
+with a 1e-3 relative-error budget sends it back to the ordinary loop, with
+retraining, when it misses), with each variant's relative L2 and max-abs
+error against the FP64 core after the same steps
+(dynamical_model_amd.errgrowth follows them over long runs).  `value` is the
+fastest of the two reduced-precision variants whose every field stays within
+the stated 1e-3 relative L2 error.
 
 Workload (default): configuration 4, 2048 x 2048 x 60, synthetic initial state
 (standard atmosphere, 20 seeded Gaussian hills, jet, RH 60 %, theta noise)
@@ -34,6 +39,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 METRIC = "column-steps/s per GPU: plain FP64 vs FP32+learned acoustic at stated L2 error"
 PROGNOSTIC = ["u", "v", "w", "pai", "tetav", "qv"]
+ERROR_BOUND = 1e-3  # the stated relative L2 error against the FP64 core
 
 
 def log(msg: str) -> None:
@@ -247,7 +253,12 @@ def main() -> int:
               for tag in results if tag != "fp64"}
     del finals
 
-    chosen_tag = "learned" if "learned" in results else ("fp32" if "fp32" in results else "fp64")
+    # value: the fastest reduced-precision variant whose every prognostic
+    # field is within the stated error (relative L2 <= ERROR_BOUND against
+    # the FP64 core after the run); FP64 when none is
+    within = {t: max(e.values()) <= ERROR_BOUND for t, e in errors.items()}
+    candidates = [t for t in ("learned", "fp32") if t in results and within.get(t)]
+    chosen_tag = max(candidates, key=lambda t: results[t]["value"]) if candidates else "fp64"
     chosen = results[chosen_tag]
     traffic = traffic_table(jx, iy, kz, "fp64" if chosen_tag == "fp64" else "fp32")
     roofline = roofline_of(chosen["_prof"], dominant(chosen["_prof"])[0], traffic)
@@ -317,6 +328,8 @@ def main() -> int:
         "rel_l2_vs_fp64": errors,
         "maxabs_vs_fp64": maxabs,
         "rel_l2_after_steps": args.lead + args.warmup + args.steps + 1,
+        "value_rule": f"fastest of fp32, learned with every field's relative L2 vs FP64 <= {ERROR_BOUND:g}",
+        "within_error_bound": within,
         "roofline": roofline,
         "stencil_roofline": stencil,
         "stencil_kernels": {k: {x: r[x] for x in ("achieved", "frac", "traffic", "avg_launch_ms", "measured_gbs")

@@ -149,6 +149,33 @@ class Moloch:
         for name in LEVELS:
             self.set_levels(name, hs.levels[name])
 
+    # ------------------------------------------------------ checkpoint/resume
+    # What a step reads of the state it continues from: the prognostic fields
+    # and the diagnostics it recomputes, as whole canonical arrays (halos
+    # included, so the nompi tree's stale halo seam carries over too).  The
+    # geometry is not part of it: a restored handle is set up from the same
+    # case first.  Values pass as FP64, which holds the FP32 state exactly.
+    STATE_FIELDS = ["u", "v", "w", "pai", "t", "tetav", "qv", "qc", "qi", "qr", "qs",
+                    "tvirt", "p", "rho", "qsat", "ux", "vx", "ps"]
+
+    def save_state(self, path: str, step: int) -> None:
+        """Raw dump of the state after step ``step`` (an .npz file)."""
+        self.sync()
+        arrays = {f: self.get_field(f) for f in self.STATE_FIELDS}
+        np.savez(path, step=np.int64(step), jx=self.jx, iy=self.iy, kz=self.kz,
+                 precision=np.int64(self.params.precision), **arrays)
+
+    def restore_state(self, path: str) -> int:
+        """Load a dump written by save_state into this handle (same grid and
+        geometry, after init_moloch or a case load); returns its step."""
+        with np.load(path) as d:
+            if (int(d["jx"]), int(d["iy"]), int(d["kz"])) != (self.jx, self.iy, self.kz):
+                raise ValueError(f"{path}: grid {int(d['jx'])}x{int(d['iy'])}x{int(d['kz'])} differs from the "
+                                 f"handle's {self.jx}x{self.iy}x{self.kz}")
+            for f in self.STATE_FIELDS:
+                self.set_field(f, d[f])
+            return int(d["step"])
+
     def load_case(self, case: _lib.Case) -> None:
         check("mlc_case_load", load().mlc_case_load(self._h, C.byref(case)))
 

@@ -70,3 +70,33 @@ def test_device_builder_is_fast():
     print(f"1024x1024x60: device build + init {td:.2f} s, host build + upload + init {th:.2f} s")
     assert td < th
     m.close()
+
+
+@pytest.mark.parametrize("precision", [dm.MLC_FP64, dm.MLC_FP32], ids=["fp64", "fp32"])
+def test_checkpoint_resume_is_bitwise(tmp_path, precision):
+    """A handle restored from the state dump after step 3 and a handle that
+    never stopped agree bit for bit after step 6."""
+    c = small_config2(48, 40, 16)
+    params = dm.case_params(c, precision=precision)
+    a = dm.Moloch()
+    a.allocate_moloch(c.jx, c.iy, c.kz, params)
+    a.load_case_device(c)
+    print(f"checkpoint 48x40x16 precision {precision}", flush=True)
+    for n in (1, 2, 3):
+        a.moloch(n)
+    path = str(tmp_path / "state.npz")
+    a.save_state(path, 3)
+    for n in (4, 5, 6):
+        a.moloch(n)
+    b = dm.Moloch()
+    b.allocate_moloch(c.jx, c.iy, c.kz, params)
+    b.load_case_device(c)  # geometry, then the state of step 3
+    assert b.restore_state(path) == 3
+    for n in (4, 5, 6):
+        b.moloch(n)
+    try:
+        for f in dm.Moloch.STATE_FIELDS:
+            assert np.array_equal(dm.interior(a.get_field(f)), dm.interior(b.get_field(f))), f
+    finally:
+        a.close()
+        b.close()
