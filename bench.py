@@ -267,15 +267,20 @@ def main() -> int:
     stencil = roofline_of(chosen["_prof"], "sound_b", traffic)
     stencils = {k: roofline_of(chosen["_prof"], k, traffic) for k in chosen["_prof"]
                 if k not in MFMA_KINDS and chosen["_prof"][k][2] > 0}
-    mlp = roofline_of(chosen["_prof"], "lac_mlp", traffic) if "lac_mlp" in chosen["_prof"] else None
+    lp = results["learned"]["_prof"] if "learned" in results else {}
+    mlp = roofline_of(lp, "lac_mlp", traffic_table(jx, iy, kz, "fp32")) if "lac_mlp" in lp else None
     # the rate of plain device-to-device copies (the corrector's phase-start
     # snapshots, hipMemcpyAsync) in the same timed window: what streaming
     # reaches on this box, next to the 8 TB/s peak
     # (the kind is charged the interior; a copy moves whole padded planes of
     # pj = roundup(jx + 34, 32) by iy + 4 points)
-    copy = chosen["_prof"].get("lac_copy")
+    copy = next((r["_prof"]["lac_copy"] for r in results.values() if "lac_copy" in r["_prof"]), None)
     plane = ((jx + 34 + 31) // 32 * 32) * (iy + 4)
     copy_gbs = copy[2] * plane / (jx * iy) / (copy[0] * 1e-3) / 1e9 if copy and copy[0] > 0 else None
+    if copy_gbs:
+        for r in [roofline, stencil] + list(stencils.values()):
+            if r.get("bound") == "hbm":
+                r["frac_of_copy"] = r["achieved"] / copy_gbs
 
     cpu = None
     serial = None
@@ -332,7 +337,8 @@ def main() -> int:
         "within_error_bound": within,
         "roofline": roofline,
         "stencil_roofline": stencil,
-        "stencil_kernels": {k: {x: r[x] for x in ("achieved", "frac", "traffic", "avg_launch_ms", "measured_gbs")
+        "stencil_kernels": {k: {x: r[x] for x in ("achieved", "frac", "frac_of_copy", "traffic", "avg_launch_ms",
+                                                  "measured_gbs")
                                 if x in r} for k, r in stencils.items()},
         "mlp_roofline": mlp,
         "copy_gbs": copy_gbs,

@@ -227,7 +227,9 @@ __global__ void __launch_bounds__(256) k_halo(F *f, Geo g, int mode) {
   }
 }
 
-// Column kernels use a (64,4) block over (j,i).
+// Column kernels use a (64,4) block over (j,i).  (Numbering the tiles so
+// that each XCD sweeps a contiguous run of them measured slower: 213 against
+// 208 ms per 2048 x 2048 x 60 FP32 step.)
 #define COL_IDX                                      \
   const int j = blockIdx.x * 64 + threadIdx.x + 1;   \
   const int i = blockIdx.y * 4 + threadIdx.y + 1;    \
@@ -1061,10 +1063,12 @@ struct FieldSet {
 // An FP64 field riding along with a group of FP32 fields (the Exner function
 // of the FP32 core): advected with every coefficient evaluated in FP64 from
 // the same loads, i.e. exactly as in a group of its own.
+// FP64 riders of a group of FP32 fields (theta and the Exner function).
+constexpr int MAXD = 2;
 struct FieldD {
-  const double *in;
-  double *wz;
-  double *out;
+  const double *in[MAXD];
+  double *wz[MAXD];
+  double *out[MAXD];
 };
 
 // Field-independent window of the vertical passes in precision R: s at
@@ -1348,7 +1352,7 @@ __device__ __forceinline__ R waf_hrow(R (&w)[5], R &zs, R pp, const HCoef<R> &c,
 template <typename T, typename Q, int NF, int ND, bool ROT, typename I>
 __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, FieldD FD, double dta, int ntj,
                                               int nti) {
-  static_assert(ND == 0 || ND == 1, "at most one FP64 rider");
+  static_assert(ND >= 0 && ND <= MAXD, "at most MAXD FP64 riders");
   const Geo &g = D.g;
   const int nj = g.nj, ni = g.ni;
   const long total = (long)ntj * nti * g.nk;  // strips
@@ -1379,7 +1383,7 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, Fi
   int i_m2 = wrap_idx(r0 - 2, ni);
   int i_m1 = nxt(i_m2), i_c = nxt(i_m1), i_p1 = nxt(i_c), i_p2 = nxt(i_p1);
   Q w[NF][5], zs[NF];
-  double wd[5], zsd = 0.0;
+  double wd[ND > 0 ? ND : 1][5], zsd[ND > 0 ? ND : 1];
   {
     // Courant number and amplitude of the south face of the first row
     const T vv = ldx(D.v, at3(jw, i_c));
@@ -1405,12 +1409,13 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, Fi
       w[f][3] = ldx(F.wz[f], at3(jw, i_p1));
       zs[f] = waf_mflux(mz, ma, w[f][0], w[f][1], w[f][2], w[f][3]);
     }
-    if (ND) {
-      wd[0] = ldx(FD.wz, at3(jw, i_m2));
-      wd[1] = ldx(FD.wz, at3(jw, i_m1));
-      wd[2] = ldx(FD.wz, at3(jw, i_c));
-      wd[3] = ldx(FD.wz, at3(jw, i_p1));
-      zsd = waf_mflux(mzd, mad, wd[0], wd[1], wd[2], wd[3]);
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      wd[d][0] = ldx(FD.wz[d], at3(jw, i_m2));
+      wd[d][1] = ldx(FD.wz[d], at3(jw, i_m1));
+      wd[d][2] = ldx(FD.wz[d], at3(jw, i_c));
+      wd[d][3] = ldx(FD.wz[d], at3(jw, i_p1));
+      zsd[d] = waf_mflux(mzd, mad, wd[d][0], wd[d][1], wd[d][2], wd[d][3]);
     }
   }
   // field-independent loads of the row centred at ic, issued a row ahead:
@@ -1442,15 +1447,16 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, Fi
   HRow<T> nx = row_in(i_c, i_p1);
   // loads of the first row
   Q nw[NF], np[NF];
-  double nwd = 0.0, npd = 0.0;
+  double nwd[ND > 0 ? ND : 1], npd[ND > 0 ? ND : 1];
 #pragma unroll
   for (int f = 0; f < NF; f++) {
     nw[f] = ldx(F.wz[f], at3(jw, i_p2));
     np[f] = ldx(F.in[f], at3(jw, i_c));
   }
-  if (ND) {
-    nwd = ldx(FD.wz, at3(jw, i_p2));
-    npd = ldx(FD.in, at3(jw, i_c));
+#pragma unroll
+  for (int d = 0; d < ND; d++) {
+    nwd[d] = ldx(FD.wz[d], at3(jw, i_p2));
+    npd[d] = ldx(FD.in[d], at3(jw, i_c));
   }
   for (int r = 0; r < nrows; r++) {
     Q pp[NF];
@@ -1459,8 +1465,12 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, Fi
       w[f][4] = nw[f];
       pp[f] = np[f];
     }
-    const double ppd = npd;
-    if (ND) wd[4] = nwd;
+    double ppd[ND > 0 ? ND : 1];
+#pragma unroll
+    for (int d = 0; d < ND; d++) {
+      ppd[d] = npd[d];
+      wd[d][4] = nwd[d];
+    }
     const HRow<T> ci = nx;
     if (r + 1 < nrows) {  // uniform over the wave
       const int i_n = nxt(i_p2);
@@ -1469,9 +1479,10 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, Fi
         nw[f] = ldx(F.wz[f], at3(jw, i_n));
         np[f] = ldx(F.in[f], at3(jw, i_p1));
       }
-      if (ND) {
-        nwd = ldx(FD.wz, at3(jw, i_n));
-        npd = ldx(FD.in, at3(jw, i_p1));
+#pragma unroll
+      for (int d = 0; d < ND; d++) {
+        nwd[d] = ldx(FD.wz[d], at3(jw, i_n));
+        npd[d] = ldx(FD.in[d], at3(jw, i_p1));
       }
       nx = row_in(i_p1, i_p2);
     }
@@ -1484,8 +1495,11 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, Fi
     }
     if (ND) {
       const HCoef<double> cd = waf_hcoef<double, ROT>(ci, v_c, rv_c, f_s, f_c, dtrdx_d, dtrdy_d);
-      const double o = waf_hrow<double, ROT>(wd, zsd, ppd, cd, lm1, lm2, lp1);
-      if (store) stx(FD.out, xo, o);
+#pragma unroll
+      for (int d = 0; d < ND; d++) {
+        const double o = waf_hrow<double, ROT>(wd[d], zsd[d], ppd[d], cd, lm1, lm2, lp1);
+        if (store) stx(FD.out[d], xo, o);
+      }
     }
     // the north values become the next row's south ones
     v_c = ci.vn;
@@ -3091,15 +3105,15 @@ struct Core : CoreBase {
   // wafone on a group of fields of one storage type: k_wafv then k_wafh; each
   // field's result buffer then takes the field's place.
   // One group: NF fields of type Q (in *fld[f], scratch wzs[f], results into
-  // outs[f], then swapped into place) and, with ND = 1, the FP64 rider
-  // *dfld (scratch dwz, results into *dout, swapped likewise).
+  // outs[f], then swapped into place) and ND FP64 riders (*dfld[d], scratch
+  // dwzs[d], results into douts[d], swapped likewise).
   // Charged bytes: s, fmz and fmzf once per advection cycle (sfm_charged),
   // and one read and one write of each advected field except the first
   // nfree of the group (ux, vx, wx, which stand in for the u, v, w passes of
   // k_adv_in and k_adv_out).
   bool sfm_charged = false;
   template <typename Q, int NF, int ND>
-  void waf_group(Q **fld[], Q **outs, Q **wzs, double **dfld, double **dout, double *dwz, double dta,
+  void waf_group(Q **fld[], Q **outs, Q **wzs, double **dfld[], double **douts, double **dwzs, double dta,
                  int nfree = 0) {
     Par P = par();
     FieldSet<Q> F;
@@ -3109,26 +3123,33 @@ struct Core : CoreBase {
       F.wz[f] = wzs[f];
       F.out[f] = outs[f];
     }
-    FieldD FD{nullptr, nullptr, nullptr};
-    if (ND) FD = FieldD{*dfld, dwz, *dout};
+    FieldD FD;
+    for (int d = 0; d < MAXD; d++) FD.in[d] = nullptr, FD.wz[d] = nullptr, FD.out[d] = nullptr;
+    for (int d = 0; d < ND; d++) {
+      FD.in[d] = *dfld[d];
+      FD.wz[d] = dwzs[d];
+      FD.out[d] = douts[d];
+    }
     const double q = sizeof(Q) == sizeof(double) ? 1.0 : 0.0;
     const int nc = NF - nfree;
-    // vertical: s, fmz, fmzf + the fields -> wz (the rider on its own)
+    // vertical: s, fmz, fmzf + the fields -> wz (the riders on their own)
     const double vb = (sfm_charged ? 0.0 : fbytes(3, 0)) + (q ? fbytes(0, nc) : fbytes(nc, 0));
     sfm_charged = true;
     if (small_offsets())
       launch(KK_WAF_V, vb, k_wafv<T, Q, NF, unsigned>, col_grid(), blk(), D, P, F, dta);
     else
       launch(KK_WAF_V, vb, k_wafv<T, Q, NF, long>, col_grid(), blk(), D, P, F, dta);
-    if (ND) {
+    if constexpr (ND > 0) {
       FieldSet<double> Fd;
       for (int f = 0; f < MAXF; f++) Fd.in[f] = nullptr, Fd.wz[f] = nullptr, Fd.out[f] = nullptr;
-      Fd.in[0] = FD.in;
-      Fd.wz[0] = FD.wz;
+      for (int d = 0; d < ND; d++) {
+        Fd.in[d] = FD.in[d];
+        Fd.wz[d] = FD.wz[d];
+      }
       if (small_offsets())
-        launch(KK_WAF_V, fbytes(0, 1), k_wafv<T, double, 1, unsigned>, col_grid(), blk(), D, P, Fd, dta);
+        launch(KK_WAF_V, fbytes(0, ND), k_wafv<T, double, ND, unsigned>, col_grid(), blk(), D, P, Fd, dta);
       else
-        launch(KK_WAF_V, fbytes(0, 1), k_wafv<T, double, 1, long>, col_grid(), blk(), D, P, Fd, dta);
+        launch(KK_WAF_V, fbytes(0, ND), k_wafv<T, double, ND, long>, col_grid(), blk(), D, P, Fd, dta);
     }
     // horizontal: u, v, fmz + the fields' wz and starting values -> results
     const int ntj = (g.nj + HJ - 1) / HJ, nti = (g.ni + HR - 1) / HR;
@@ -3147,22 +3168,22 @@ struct Core : CoreBase {
         launch(KK_WAF_H, hb, k_wafh<T, Q, NF, ND, false, long>, hg, dim3(256), D, P, F, FD, dta, ntj, nti);
     }
     for (int f = 0; f < NF; f++) std::swap(*fld[f], outs[f]);
-    if (ND) std::swap(*dfld, *dout);
+    for (int d = 0; d < ND; d++) std::swap(*dfld[d], douts[d]);
   }
-  // The n fields in groups of at most MAXF; the FP64 rider, if any, goes
+  // The n fields in groups of at most MAXF; the nd FP64 riders, if any, go
   // with the last group.
   // The first nfree fields are not charged (see waf_group).
   template <typename Q>
-  void waf_groups(Q **fld[], int n, Q **outs, Q **wzs, double **dfld, double **dout, double *dwz, double dta,
-                  int nfree = 0) {
+  void waf_groups(Q **fld[], int n, Q **outs, Q **wzs, double **dfld[], int nd, double **douts, double **dwzs,
+                  double dta, int nfree = 0) {
     for (int b = 0; b < n; b += MAXF) {
       const int m = n - b < MAXF ? n - b : MAXF;
-      const bool rider = dfld && b + m == n;
+      const bool rider = nd > 0 && b + m == n;
       const int nf = nfree - b < 0 ? 0 : (nfree - b > m ? m : nfree - b);
 #define MLC_WAF_GROUP(NF)                                                   \
   if constexpr (sizeof(Q) != sizeof(double)) {                              \
     if (rider) {                                                            \
-      waf_group<Q, NF, 1>(fld + b, outs, wzs, dfld, dout, dwz, dta, nf);   \
+      waf_group<Q, NF, MAXD>(fld + b, outs, wzs, dfld, douts, dwzs, dta, nf); \
       break;                                                                \
     }                                                                       \
   }                                                                         \
@@ -3197,12 +3218,14 @@ struct Core : CoreBase {
       list[n++] = &D.qv;
       for (int s = prm.iqfrst; s <= dims.nqx; s++) list[n++] = qm[s - 1];
       sfm_charged = false;
-      waf_groups<T>(list, n, gout, gwz, nullptr, nullptr, nullptr, dta, 3);
       if constexpr (sizeof(T) != sizeof(double)) {
-        // theta and the Exner function: a group of their own in FP64
-        double **dl[2] = {&D.tetav, &D.pai};
-        double *dwz[2] = {D.p0, D.wz};
-        waf_group<double, 2, 0>(dl, dgout, dwz, nullptr, nullptr, nullptr, dta);
+        // theta and the Exner function (FP64) ride with the last group
+        static_assert(MAXD == 2, "two FP64 riders");
+        double **dl[MAXD] = {&D.tetav, &D.pai};
+        double *dwz[MAXD] = {D.p0, D.wz};
+        waf_groups<T>(list, n, gout, gwz, dl, MAXD, dgout, dwz, dta, 3);
+      } else {
+        waf_groups<T>(list, n, gout, gwz, nullptr, 0, nullptr, nullptr, dta, 3);
       }
       halo(D.ux, g.nk, 2);
       halo(D.vx, g.nk, 3);
