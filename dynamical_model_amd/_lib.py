@@ -111,6 +111,7 @@ EXPORTS = {
     "mlc_init": (C.c_int, [C.c_void_p]),
     "mlc_step": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(Diag)]),
     "mlc_sync": (C.c_int, [C.c_void_p]),
+    "mlc_debug_poison_halos": (C.c_int, [C.c_void_p, C.c_int]),
     "mlc_destroy": (C.c_int, [C.c_void_p]),
     "mlc_last_error": (C.c_char_p, []),
     "mlc_case_config": (C.c_int, [C.c_int, C.POINTER(Case)]),

@@ -186,6 +186,10 @@ class Moloch:
     def sync(self) -> None:
         check("mlc_sync", load().mlc_sync(self._h))
 
+    def poison_halos(self, enable: bool = True) -> None:
+        """Debug: NaN into every state and scratch halo before each step."""
+        check("mlc_debug_poison_halos", load().mlc_debug_poison_halos(self._h, 1 if enable else 0))
+
     def profile(self, enable: bool) -> None:
         """Bracket every kernel launch with HIP events (and clear the totals)."""
         check("mlc_profile", load().mlc_profile(self._h, 1 if enable else 0))

@@ -2584,6 +2584,27 @@ __global__ void __launch_bounds__(256) k_periodic(F *f, Geo g) {
 // ============================================================ output packing
 // Interior points of a field, j fastest, as doubles: the layout of the
 // reference's output records (nompi/mod_mppparam.F90:197,310).
+// Debug: a quiet NaN into every cell of the two-point halo ring of level k
+// (rows i = -1, 0, ni+1, ni+2 and columns j = -1, 0, nj+1, nj+2).
+template <typename F>
+__global__ void __launch_bounds__(256) k_poison(F *f, Geo g) {
+  const int t = blockIdx.x * 256 + threadIdx.x, k = blockIdx.y + 1;
+  const int ncol = 4 * (g.ni + 4), nrow = 4 * g.nj;
+  int j, i;
+  if (t < ncol) {
+    const int c = t % 4;
+    i = t / 4 - 1;
+    j = c < 2 ? c - 1 : g.nj + c - 1;
+  } else if (t < ncol + nrow) {
+    const int r = (t - ncol) % 4;
+    j = (t - ncol) / 4 + 1;
+    i = r < 2 ? r - 1 : g.ni + r - 1;
+  } else {
+    return;
+  }
+  f[g.at(j, i, k)] = (F)__builtin_nan("");
+}
+
 template <typename F>
 __global__ void __launch_bounds__(256) k_pack_interior(const F *src, Geo g, double *dst) {
   const int j = blockIdx.x * 64 + threadIdx.x % 64 + 1;
@@ -2612,6 +2633,7 @@ struct CoreBase {
   virtual int init() = 0;
   virtual int step(int istep, int lprint, mlc_diag *diag) = 0;
   virtual int sync() = 0;
+  virtual int poison_halos(int enable) = 0;
   virtual int profile(int enable) = 0;
   virtual int profile_get(int kind, double *total_ms, long long *launches, double *bytes) = 0;
   virtual int lac_create(const mlc_lac_config &c) = 0;
@@ -3786,6 +3808,25 @@ struct Core : CoreBase {
     return run(lprint, diag, false);
   }
 
+  // Debug mode: before every step, NaN into the halo ring of every state and
+  // scratch field (not the geometry), so a read of a halo cell the step has
+  // not filled first shows up as a NaN in the interior.
+  bool poison = false;
+  int poison_halos(int enable) override {
+    poison = enable != 0;
+    return MLC_OK;
+  }
+  void poison_now() {
+    const int K = g.nk;
+    const dim3 pg((unsigned)((4 * (g.ni + 4) + 4 * g.nj + 255) / 256), 1);
+    T *tf[] = {D.u, D.v, D.qv, D.qc, D.qi, D.qr, D.qs, D.ux, D.vx, D.wx, D.zdh, D.tvirt, D.p, D.rho, D.qsat};
+    for (T *f : tf) hipLaunchKernelGGL(k_poison<T>, dim3(pg.x, K), dim3(256), 0, stream, f, g);
+    T *tf1[] = {D.w, D.s, D.deltaw};
+    for (T *f : tf1) hipLaunchKernelGGL(k_poison<T>, dim3(pg.x, K + 1), dim3(256), 0, stream, f, g);
+    double *df[] = {D.pai, D.tetav, D.t};
+    for (double *f : df) hipLaunchKernelGGL(k_poison<double>, dim3(pg.x, K), dim3(256), 0, stream, f, g);
+  }
+
   // collect: record training pairs in every acoustic phase (ordinary loop).
   int run(int lprint, mlc_diag *diag, bool collect) {
     if (!initialised) return mlc_fail(MLC_ESTATE, "mlc_step before mlc_init");
@@ -3800,6 +3841,7 @@ struct Core : CoreBase {
     if (learned && !L.normalised) return mlc_fail(MLC_ESTATE, "mlc_step: the corrector is not trained");
     const bool check = learned && L.cfg.check_every > 0 && L.learned_steps % L.cfg.check_every == 0;
     Par P = par();
+    if (poison) poison_now();
     launch(KK_PROLOGUE, fbytes(8, 3), k_prologue<T>, col_grid(), blk(), D, P, prm.filterpai ? 1 : 0,
            (prm.fulleq && prm.filtertheta) ? 1 : 0);
     double worst = 0.0, worst_rel = 0.0;
@@ -4012,6 +4054,11 @@ extern "C" int mlc_step(mlc_handle *h, int istep, int lprint, mlc_diag *diag) {
 extern "C" int mlc_sync(mlc_handle *h) {
   if (!h) return mlc_fail(MLC_EINVAL, "mlc_sync: NULL handle");
   return h->core->sync();
+}
+
+extern "C" int mlc_debug_poison_halos(mlc_handle *h, int enable) {
+  if (!h) return mlc_fail(MLC_EINVAL, "mlc_debug_poison_halos: NULL handle");
+  return h->core->poison_halos(enable);
 }
 
 extern "C" int mlc_profile(mlc_handle *h, int enable) {

@@ -124,6 +124,8 @@ def main(argv=None) -> int:
         for m in runs.values():
             m.moloch(n, False)
         k = n - args.lead
+        if k % 50 == 0:  # a progress line at least every 50 steps
+            print(json.dumps({"progress": n, "elapsed_s": round(time.perf_counter() - t0, 1)}), flush=True)
         due = (k in report_at) if report_at else (k % args.every == 0)
         if due or n == args.lead + args.steps:
             if ref is not None:

@@ -170,6 +170,13 @@ int mlc_step(mlc_handle *h, int istep, int lprint, mlc_diag *diag);
 /* Wait for all work queued on the handle's stream. */
 int mlc_sync(mlc_handle *h);
 
+/* Debug mode (no counterpart in the reference, which has no halo checks):
+ * with enable != 0, every mlc_step first writes NaN into the two-point halo
+ * ring of every state and scratch field (the geometry excepted), so that a
+ * kernel reading a halo cell the step has not refilled puts a NaN into the
+ * interior.  The step's results are otherwise unchanged. */
+int mlc_debug_poison_halos(mlc_handle *h, int enable);
+
 /* Per-kernel timing for benchmarks (no counterpart in the reference, whose
  * only timer is whole-run cpu_time, nompi/regcm_dynamical_core.F90:30,222-228).
  * mlc_profile(h, 1) brackets every subsequent kernel launch of the handle

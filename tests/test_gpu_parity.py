@@ -312,3 +312,33 @@ def test_case_load_matches_host_build():
         got = m.get_field(name)
         np.testing.assert_array_equal(dm.interior(got), dm.interior(hs.fields[name]), err_msg=name)
     m.close()
+
+
+@pytest.mark.parametrize("precision", [dm.MLC_FP64, dm.MLC_FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("halo_mpi", [0, 1], ids=["nompi", "mpi"])
+def test_poisoned_halos_change_nothing(precision, halo_mpi):
+    """Debug mode of the halo fills: NaN in the two-point halo ring of every
+    state and scratch field before each step.  Every halo cell a kernel reads
+    is refilled in the step before it is read (the nompi tree's stale Exner
+    seam included: its halo holds the substep's pre-update values, written
+    in the same substep), so the interior comes out finite and bit-identical
+    to the run without poison."""
+    c = small_config2(70, 45, 20)
+    hs = dm.build_case(c)
+    params = dm.case_params(c, precision=precision, halo_mpi=halo_mpi)
+    print(f"poisoned halos 70x45x20 precision {precision} halo_mpi {halo_mpi}", flush=True)
+    out = []
+    for poison in (False, True):
+        m = dm.Moloch()
+        m.allocate_moloch(c.jx, c.iy, c.kz, params)
+        m.load_state(hs)
+        m.init_moloch()
+        m.poison_halos(poison)
+        for n in range(1, 4):
+            m.moloch(n)
+        m.sync()
+        out.append({f: dm.interior(m.get_field(f)).copy() for f in PROGNOSTIC + ["t", "ps"]})
+        m.close()
+    for f in out[0]:
+        assert np.isfinite(out[1][f]).all(), f
+        assert np.array_equal(out[0][f], out[1][f]), f
