@@ -3821,8 +3821,12 @@ struct Core : CoreBase {
     const dim3 pg((unsigned)((4 * (g.ni + 4) + 4 * g.nj + 255) / 256), 1);
     T *tf[] = {D.u, D.v, D.qv, D.qc, D.qi, D.qr, D.qs, D.ux, D.vx, D.wx, D.zdh, D.tvirt, D.p, D.rho, D.qsat};
     for (T *f : tf) hipLaunchKernelGGL(k_poison<T>, dim3(pg.x, K), dim3(256), 0, stream, f, g);
-    T *tf1[] = {D.w, D.s, D.deltaw};
+    T *tf1[] = {D.w, D.s};
     for (T *f : tf1) hipLaunchKernelGGL(k_poison<T>, dim3(pg.x, K + 1), dim3(256), 0, stream, f, g);
+    // deltaw at levels 2..kz only: levels 1 and kz+1 (halos included) are the
+    // zeros reset_tendencies leaves there for good (:1313-1325), which the
+    // momentum equations read at the bottom level (:748-750, :773-775)
+    if (K > 1) hipLaunchKernelGGL(k_poison<T>, dim3(pg.x, K - 1), dim3(256), 0, stream, D.deltaw + g.plane, g);
     double *df[] = {D.pai, D.tetav, D.t};
     for (double *f : df) hipLaunchKernelGGL(k_poison<double>, dim3(pg.x, K), dim3(256), 0, stream, f, g);
   }

@@ -114,9 +114,11 @@ def main(argv):
         fb = fm["mlp_fused"] + per * fm["mlp_rest"]
         wb = wm["mlp_fused"] + per * wm["mlp_rest"]
         kinds["lac_mlp"] = {"fetch_bytes": fb, "write_bytes": wb, "bytes": fb + wb, "dispatches": fc["mlp_fused"]}
+    toks = args.split()
+    config = toks[toks.index("--config") + 1] if "--config" in toks else "4"
     out = {
-        "workload": f"configuration 4, {jx}x{iy}x{kz}, FP32 state (FP64 Exner): kernels of the FP32 and "
-                    "FP32+learned steps",
+        "workload": f"configuration {config}, {jx}x{iy}x{kz}, FP32 state (FP64 Exner function, theta and "
+                    "temperature): kernels of the FP32 and FP32+learned steps",
         "commands": [f"rocprofv3 --pmc FETCH_SIZE --output-format csv -d prof -o run -- python3 bench.py {args}",
                      f"rocprofv3 --pmc WRITE_SIZE --output-format csv -d prof -o run -- python3 bench.py {args}",
                      f"python3 -m dynamical_model_amd.traffic <fetch dir> <write dir> {jx} {iy} {kz} '{args}'"],
