@@ -1,5 +1,6 @@
 """The device output path: ps.nc and t.nc written from a handle while the
-steps go on, read back with scipy and compared with the fields."""
+steps go on, read back with scipy and compared with the fields and, in FP64,
+with the CPU oracle's ps and t at the same steps."""
 from __future__ import annotations
 
 import numpy as np
@@ -7,6 +8,7 @@ import pytest
 from scipy.io import netcdf_file
 
 import dynamical_model_amd as dm
+import oracle_lib
 from dynamical_model_amd.output import Output
 from parity import small_config2
 
@@ -42,3 +44,14 @@ def test_output_records_match_fields(tmp_path, precision):
     # the records differ: the state moved between them
     assert not np.array_equal(t[0], t[2])
     m.close()
+    if precision == dm.MLC_FP64:
+        # the file's content against the CPU oracle's ps and t after 0, 2 and
+        # 4 steps (t bit for bit; ps through the device exp, to the last ulp)
+        o = oracle_lib.oracle_from_host(hs, dm.case_params(case))
+        done = 0
+        for r, nsteps in enumerate((0, 2, 4)):
+            while done < nsteps:
+                done += 1
+                o.moloch(done, False)
+            np.testing.assert_array_equal(t[r], dm.interior(o.field("t")))
+            np.testing.assert_allclose(ps[r], dm.interior(o.field("ps")), rtol=1e-14, atol=0)
