@@ -170,6 +170,21 @@ __device__ __forceinline__ R waf_b(R r) {
   m = fmin(R(2), m);
   return fmax(R(0), m);
 }
+// Quotient of the advection's limiter ratios and metric coefficients: IEEE
+// division in FP64 (the FP64 core and the FP64 riders keep the oracle's
+// rounding); in FP32 the numerator times the hardware reciprocal (1 ulp), a
+// quotient within 2 ulp of the rounded one in two instructions instead of
+// the eleven of the correctly rounded sequence (scale, reciprocal, four
+// refinements, fixup).  Denominators here are at least kMinden (1e-30) in
+// magnitude, far from the reciprocal's denormal range.
+template <typename R>
+__device__ __forceinline__ R wdiv(R a, R b) {
+  return a / b;
+}
+template <>
+__device__ __forceinline__ float wdiv<float>(float a, float b) {
+  return a * __builtin_amdgcn_rcpf(b);
+}
 template <typename R>
 __device__ __forceinline__ R sgn_floor(R z) {
   return copysign(fmax(fabs(z), R(kMinden)), z);
@@ -873,14 +888,14 @@ __device__ __forceinline__ void waf_vpass(const Dev<T> &D, const Geo &g, int j, 
         b = q0;                      // level k
       }
       Q zzden = sgn_floor(q0 - q1);
-      Q r = (a - b) / zzden;
+      Q r = wdiv(a - b, zzden);
       Q bb = waf_b(r);
       Q zphi = is + zamu * bb - is * bb;
       wfw_k1 = Q(0.5) * sk1 * ((Q(1) + zphi) * q1 + (Q(1) - zphi) * q0);
     }
     Q f0 = (Q)D.fmz[x];
-    Q zrfmu = dtrdz * f0 / (Q)D.fmzf[x];
-    Q zrfmd = dtrdz * f0 / (Q)D.fmzf[x + g.plane];
+    Q zrfmu = wdiv(dtrdz * f0, (Q)D.fmzf[x]);
+    Q zrfmd = wdiv(dtrdz * f0, (Q)D.fmzf[x + g.plane]);
     Q zdv = (sk * zrfmu - sk1 * zrfmd) * q0;
     Q out = q0 - wfw_k * zrfmu + wfw_k1 * zrfmd + zdv;
     // advance the window before the in-place store of level k
@@ -938,7 +953,7 @@ __global__ void __launch_bounds__(256) k_waf_m(Dev<T> D, Par P, const Q *pp, con
     int ihm1 = ih - 1 > imin ? ih - 1 : imin;
     Q wc = wz[xx], wm = wz[xx - g.pj];
     Q zzden = sgn_floor(wc - wm);
-    Q r = (wz[g.at(j, ih, k)] - wz[g.at(j, ihm1, k)]) / zzden;
+    Q r = wdiv(wz[g.at(j, ih, k)] - wz[g.at(j, ihm1, k)], zzden);
     Q b = waf_b(r);
     Q zphi = is + zamu * b - is * b;
     if (P.rotllr) return Q(0.5) * vv * ((Q(1) + zphi) * wm + (Q(1) - zphi) * wc);
@@ -952,13 +967,13 @@ __global__ void __launch_bounds__(256) k_waf_m(Dev<T> D, Par P, const Q *pp, con
     Q mx = (Q)D.mx[g.at2(j, i)];
     Q zhxvtn = dtrdy * (Q)D.rmv[g.at2(j, i + 1)] * mx;
     Q zhxvts = dtrdy * (Q)D.rmv[g.at2(j, i)] * mx;
-    Q zrfmn = zhxvtn * Q(2) * f0 / (f0 + (Q)D.fmz[x + g.pj]);
-    Q zrfms = zhxvts * Q(2) * f0 / (f0 + (Q)D.fmz[x - g.pj]);
+    Q zrfmn = wdiv(zhxvtn * Q(2) * f0, f0 + (Q)D.fmz[x + g.pj]);
+    Q zrfms = wdiv(zhxvts * Q(2) * f0, f0 + (Q)D.fmz[x - g.pj]);
     Q zdv = ((Q)D.v[x + g.pj] * zrfmn - (Q)D.v[x] * zrfms) * pp[x];
     out = wz[x] + zs * zrfms - zn * zrfmn + zdv;
   } else {
-    Q zrfmn = dtrdy * Q(2) * f0 / (f0 + (Q)D.fmz[x + g.pj]);
-    Q zrfms = dtrdy * Q(2) * f0 / (f0 + (Q)D.fmz[x - g.pj]);
+    Q zrfmn = wdiv(dtrdy * Q(2) * f0, f0 + (Q)D.fmz[x + g.pj]);
+    Q zrfms = wdiv(dtrdy * Q(2) * f0, f0 + (Q)D.fmz[x - g.pj]);
     Q zdv = ((Q)D.v[x + g.pj] * (Q)D.rmv[g.at2(j, i + 1)] * zrfmn - (Q)D.v[x] * (Q)D.rmv[g.at2(j, i)] * zrfms) * pp[x];
     out = wz[x] + (Q)D.mx2[g.at2(j, i)] * (zs * zrfms - zn * zrfmn + zdv);
   }
@@ -996,7 +1011,7 @@ __global__ void __launch_bounds__(256) k_waf_z(Dev<T> D, Par P, Q *pp, const Q *
     int jhm1 = jh - 1 > jmin ? jh - 1 : jmin;
     Q pc = p0[xx], pm = p0[xx - 1];
     Q zzden = sgn_floor(pc - pm);
-    Q r = (p0[g.at(jh, i, k)] - p0[g.at(jhm1, i, k)]) / zzden;
+    Q r = wdiv(p0[g.at(jh, i, k)] - p0[g.at(jhm1, i, k)], zzden);
     Q b = waf_b(r);
     Q zphi = is + zamu * b - is * b;
     if (P.rotllr) return Q(0.5) * uu * ((Q(1) + zphi) * pm + (Q(1) - zphi) * pc);
@@ -1008,13 +1023,13 @@ __global__ void __launch_bounds__(256) k_waf_z(Dev<T> D, Par P, Q *pp, const Q *
   Q out;
   if (P.rotllr) {
     Q zcostx = dtrdx * (Q)D.mu[g.at2(j, i)];
-    Q zrfme = zcostx * Q(2) * f0 / (f0 + (Q)D.fmz[x + 1]);
-    Q zrfmw = zcostx * Q(2) * f0 / (f0 + (Q)D.fmz[x - 1]);
+    Q zrfme = wdiv(zcostx * Q(2) * f0, f0 + (Q)D.fmz[x + 1]);
+    Q zrfmw = wdiv(zcostx * Q(2) * f0, f0 + (Q)D.fmz[x - 1]);
     Q zdv = ((Q)D.u[x + 1] * zrfme - (Q)D.u[x] * zrfmw) * pp[x];
     out = p0[x] + zw * zrfmw - ze * zrfme + zdv;
   } else {
-    Q zrfme = dtrdx * Q(2) * f0 / (f0 + (Q)D.fmz[x + 1]);
-    Q zrfmw = dtrdx * Q(2) * f0 / (f0 + (Q)D.fmz[x - 1]);
+    Q zrfme = wdiv(dtrdx * Q(2) * f0, f0 + (Q)D.fmz[x + 1]);
+    Q zrfmw = wdiv(dtrdx * Q(2) * f0, f0 + (Q)D.fmz[x - 1]);
     Q zdv = ((Q)D.u[x + 1] * (Q)D.rmu[g.at2(j + 1, i)] * zrfme - (Q)D.u[x] * (Q)D.rmu[g.at2(j, i)] * zrfmw) * pp[x];
     out = p0[x] + (Q)D.mx2[g.at2(j, i)] * (zw * zrfmw - ze * zrfme + zdv);
   }
@@ -1036,7 +1051,7 @@ __device__ __forceinline__ Q waf_vflux(Q s, Q dtrdz, Q qm2, Q qm1, Q q0, Q q1) {
     b = qm1;
   }
   Q zzden = sgn_floor(qm1 - q0);
-  Q r = (a - b) / zzden;
+  Q r = wdiv(a - b, zzden);
   Q bb = waf_b(r);
   Q zphi = is + zamu * bb - is * bb;
   return Q(0.5) * s * ((Q(1) + zphi) * q0 + (Q(1) - zphi) * qm1);
@@ -1087,8 +1102,8 @@ struct VCoef {
     for (int q = 0; q < 4; q++) S[q] = S[q + 1], ZU[q] = ZU[q + 1], ZD[q] = ZD[q + 1];
     S[4] = sm;
     if (m <= nk) {
-      ZU[4] = dtrdz * fzm / ffz;  // zrfmu (:1010)
-      ZD[4] = dtrdz * fzm / ff1;  // zrfmd (:1011)
+      ZU[4] = wdiv(dtrdz * fzm, ffz);  // zrfmu (:1010)
+      ZD[4] = wdiv(dtrdz * fzm, ff1);  // zrfmd (:1011)
       ffz = ff1;
     }
   }
@@ -1233,7 +1248,7 @@ __device__ __forceinline__ R waf_mflux(R mz, R ma, R wmm, R wm, R wc, R wp) {
     b = wc;
   }
   const R zzden = sgn_floor(wc - wm);
-  const R rq = (a - b) / zzden;
+  const R rq = wdiv(a - b, zzden);
   const R bb = waf_b(rq);
   const R zphi = is + mz * bb - is * bb;
   return ma * ((R(1) + zphi) * wm + (R(1) - zphi) * wc);
@@ -1267,13 +1282,13 @@ __device__ __forceinline__ HCoef<R> waf_hcoef(const HRow<T> &ci, T v_c, T rv_c, 
       const R mx = (R)ci.m0;
       const R zhxvtn = dtrdy * rvn * mx;
       const R zhxvts = dtrdy * (R)rv_c * mx;
-      c.pfn = zhxvtn * R(2) * f0 / (f0 + fn);
-      c.pfs = zhxvts * R(2) * f0 / (f0 + fs);
+      c.pfn = wdiv(zhxvtn * R(2) * f0, f0 + fn);
+      c.pfs = wdiv(zhxvts * R(2) * f0, f0 + fs);
       c.pcd = vp * c.pfn - vc * c.pfs;
       c.pm2 = R(1);
     } else {
-      c.pfn = dtrdy * R(2) * f0 / (f0 + fn);
-      c.pfs = dtrdy * R(2) * f0 / (f0 + fs);
+      c.pfn = wdiv(dtrdy * R(2) * f0, f0 + fn);
+      c.pfs = wdiv(dtrdy * R(2) * f0, f0 + fs);
       c.pcd = vp * rvn * c.pfn - vc * (R)rv_c * c.pfs;
       c.pm2 = (R)ci.m0;
     }
@@ -1286,16 +1301,16 @@ __device__ __forceinline__ HCoef<R> waf_hcoef(const HRow<T> &ci, T v_c, T rv_c, 
       c.zz = uc * mu * dtrdx;
       c.za = R(0.5) * uc;
       const R zcostx = dtrdx * mu;
-      c.zfe = zcostx * R(2) * f0 / (f0 + fe);
-      c.zfw = zcostx * R(2) * f0 / (f0 + fw);
+      c.zfe = wdiv(zcostx * R(2) * f0, f0 + fe);
+      c.zfw = wdiv(zcostx * R(2) * f0, f0 + fw);
       c.zcd = up * c.zfe - uc * c.zfw;
       c.zm2 = R(1);
     } else {
       const R rm = (R)ci.m1;
       c.zz = uc * rm * dtrdx;
       c.za = R(0.5) * uc * rm;
-      c.zfe = dtrdx * R(2) * f0 / (f0 + fe);
-      c.zfw = dtrdx * R(2) * f0 / (f0 + fw);
+      c.zfe = wdiv(dtrdx * R(2) * f0, f0 + fe);
+      c.zfw = wdiv(dtrdx * R(2) * f0, f0 + fw);
       c.zcd = up * (R)ci.m2 * c.zfe - uc * rm * c.zfw;
       c.zm2 = (R)ci.m0;
     }
@@ -1327,7 +1342,7 @@ __device__ __forceinline__ R waf_hrow(R (&w)[5], R &zs, R pp, const HCoef<R> &c,
     b = p0;
   }
   const R zzden = sgn_floor(p0 - pm);
-  const R rq = (a - b) / zzden;
+  const R rq = wdiv(a - b, zzden);
   const R bb = waf_b(rq);
   const R zphi = is + c.zz * bb - is * bb;
   const R zw = c.za * ((R(1) + zphi) * pm + (R(1) - zphi) * p0);
