@@ -251,6 +251,13 @@ __global__ void __launch_bounds__(256) k_halo(F *f, Geo g, int mode) {
   if (j > D.g.nj || i > D.g.ni) return;              \
   const Geo &g = D.g;                                \
   (void)g;
+// COL_IDX for (64, BY) blocks
+#define COL_IDX_B(BY)                                \
+  const int j = blockIdx.x * 64 + threadIdx.x + 1;   \
+  const int i = blockIdx.y * (BY) + threadIdx.y + 1; \
+  if (j > D.g.nj || i > D.g.ni) return;              \
+  const Geo &g = D.g;                                \
+  (void)g;
 
 // ================================================================ prologue
 // :291-323 -- p and qsat, tvirt, qwltot/qwitot, tetav; filter copies.
@@ -685,9 +692,13 @@ __global__ void __launch_bounds__(256) k_sound_b2(Dev<T> D, Par P) {
 // hold the substep-start winds and each interior value read gets divdamp's
 // increment (:865-872) here, from the horizontal divergence zdiv2 of phase
 // A, in the operand order of k_sound_b, so the winds agree bit for bit.
-template <typename T, bool DAMP, typename I>
-__global__ void __launch_bounds__(256) k_sound_c(Dev<T> D, Par P, double dts) {
-  COL_IDX
+// BY: rows per (64, BY) block (16 for the FP32 state: the rows above and
+// below a block are read once per 16 rows instead of once per 4, 2.46
+// against 2.73 ms per launch at 2048 x 2048 x 60; the FP64 kernel needs more
+// than the 128 VGPRs of a 1024-lane block and keeps 4).
+template <typename T, bool DAMP, typename I, int BY>
+__global__ void __launch_bounds__(64 * BY) k_sound_c(Dev<T> D, Par P, double dts) {
+  COL_IDX_B(BY)
   const double dtrdx = dts / P.dx, dtrdy = dts / P.dx;
   // element indices in I (unsigned: 32-bit byte offsets from the SGPR bases)
   const I c2 = (I)g.at2(j, i);
@@ -3053,6 +3064,7 @@ struct Core : CoreBase {
   dim3 col_grid() const { return dim3((g.nj + 63) / 64, (g.ni + 3) / 4, 1); }
   dim3 pt_grid() const { return dim3((g.nj + 63) / 64, (g.ni + 3) / 4, g.nk); }
   dim3 blk() const { return dim3(64, 4, 1); }
+  dim3 col_grid(int by) const { return dim3((g.nj + 63) / 64, (g.ni + by - 1) / by, 1); }
   Par par() const {
     Par P;
     P.dx = prm.dx;
@@ -3102,13 +3114,13 @@ struct Core : CoreBase {
     // slower with them (79 against 63 ms per 2048 x 2048 x 60 step)
     const bool so = sizeof(T) == sizeof(float) && small_offsets();
     if (seg && so)
-      launch(KK_SOUND_C, fbytes(5, 2), k_sound_c<T, true, unsigned>, col_grid(), blk(), D, P, dts);
+      launch(KK_SOUND_C, fbytes(5, 2), k_sound_c<T, true, unsigned, 16>, col_grid(16), dim3(64, 16), D, P, dts);
     else if (seg)
-      launch(KK_SOUND_C, fbytes(5, 2), k_sound_c<T, true, long>, col_grid(), blk(), D, P, dts);
+      launch(KK_SOUND_C, fbytes(5, 2), k_sound_c<T, true, long, 4>, col_grid(), blk(), D, P, dts);
     else if (so)
-      launch(KK_SOUND_C, fbytes(5, 2), k_sound_c<T, false, unsigned>, col_grid(), blk(), D, P, dts);
+      launch(KK_SOUND_C, fbytes(5, 2), k_sound_c<T, false, unsigned, 16>, col_grid(16), dim3(64, 16), D, P, dts);
     else
-      launch(KK_SOUND_C, fbytes(5, 2), k_sound_c<T, false, long>, col_grid(), blk(), D, P, dts);
+      launch(KK_SOUND_C, fbytes(5, 2), k_sound_c<T, false, long, 4>, col_grid(), blk(), D, P, dts);
     std::swap(D.u, D.un);
     std::swap(D.v, D.vn);
   }
