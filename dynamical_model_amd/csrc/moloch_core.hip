@@ -834,41 +834,46 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_adv_in(Dev<T> D) {
   COL_IDX
   const int nk = g.nk;
-  for (int k = 2; k <= nk; k++) {
-    long x = g.at(j, i, k);
-    D.s[x] = (D.w[x] + D.s[x]) * D.fmzf[x];
-  }
-  D.s[g.at(j, i, 1)] = T(0);
-  D.s[g.at(j, i, nk + 1)] = T(0);
-  for (int k = 2; k <= nk - 1; k++) {
-    long x = g.at(j, i, k);
-    D.wx[x] = T(0.5625) * (D.w[x + g.plane] + D.w[x]) - T(0.0625) * (D.w[x + 2 * g.plane] + D.w[x - g.plane]);
-  }
-  D.wx[g.at(j, i, 1)] = T(0.5) * (D.w[g.at(j, i, 2)] + D.w[g.at(j, i, 1)]);
-  D.wx[g.at(j, i, nk)] = T(0.5) * (D.w[g.at(j, i, nk + 1)] + D.w[g.at(j, i, nk)]);
+  const long pl = g.plane;
+  // one pass down the column, w in a window of levels k-1 .. k+2
+  const long x1 = g.at(j, i, 1);
+  T wm = D.w[x1], w0 = wm, w1 = D.w[x1 + pl], w2 = nk >= 2 ? D.w[x1 + 2 * pl] : w1;
   for (int k = 1; k <= nk; k++) {
-    long x = g.at(j, i, k);
+    const long x = x1 + (long)(k - 1) * pl;
+    if (k >= 2) D.s[x] = (w0 + D.s[x]) * D.fmzf[x];
+    if (k == 1)
+      D.wx[x] = T(0.5) * (w1 + w0);
+    else if (k == nk)
+      D.wx[x] = T(0.5) * (w1 + w0);
+    else
+      D.wx[x] = T(0.5625) * (w1 + w0) - T(0.0625) * (w2 + wm);
     D.ux[x] = T(0.5625) * (D.u[x + 1] + D.u[x]) - T(0.0625) * (D.u[x + 2] + D.u[x - 1]);
     D.vx[x] = T(0.5625) * (D.v[x + g.pj] + D.v[x]) - T(0.0625) * (D.v[x + 2 * g.pj] + D.v[x - g.pj]);
+    wm = w0;
+    w0 = w1;
+    w1 = w2;
+    if (k + 3 <= nk + 1) w2 = D.w[x + 3 * pl];
   }
+  D.s[x1] = T(0);
+  D.s[g.at(j, i, nk + 1)] = T(0);
 }
 
 // xtouvstag (:1374-1386) and xtowstag (:1343-1357); ux/vx carry halos.
+// kz >= 3 (mlc_create).
 template <typename T>
 __global__ void __launch_bounds__(256) k_adv_out(Dev<T> D) {
   COL_IDX
   const int nk = g.nk;
+  const long pl = g.plane;
+  // one pass down the column
   for (int k = 1; k <= nk; k++) {
-    long x = g.at(j, i, k);
+    const long x = g.at(j, i, k);
     D.u[x] = T(0.5625) * (D.ux[x] + D.ux[x - 1]) - T(0.0625) * (D.ux[x + 1] + D.ux[x - 2]);
     D.v[x] = T(0.5625) * (D.vx[x] + D.vx[x - g.pj]) - T(0.0625) * (D.vx[x + g.pj] + D.vx[x - 2 * g.pj]);
+    if (k == 2) D.w[x] = T(0.5) * (D.wx[x] + D.wx[x - pl]);
+    if (k >= 3 && k <= nk - 1) D.w[x] = T(0.5625) * (D.wx[x] + D.wx[x - pl]) - T(0.0625) * (D.wx[x + pl] + D.wx[x - 2 * pl]);
+    if (k == nk && k >= 3) D.w[x] = T(0.5) * (D.wx[x] + D.wx[x - pl]);
   }
-  for (int k = 3; k <= nk - 1; k++) {
-    long x = g.at(j, i, k);
-    D.w[x] = T(0.5625) * (D.wx[x] + D.wx[x - g.plane]) - T(0.0625) * (D.wx[x + g.plane] + D.wx[x - 2 * g.plane]);
-  }
-  D.w[g.at(j, i, 2)] = T(0.5) * (D.wx[g.at(j, i, 2)] + D.wx[g.at(j, i, 1)]);
-  D.w[g.at(j, i, nk)] = T(0.5) * (D.wx[g.at(j, i, nk)] + D.wx[g.at(j, i, nk - 1)]);
 }
 
 // ================================================================= wafone
@@ -1544,10 +1549,13 @@ __global__ void __launch_bounds__(256) k_wafh(Dev<T> D, Par P, FieldSet<Q> F, Fi
 // :355-391 -- tvirt, t, p, rho, ps (level-kz extrapolation), qsat.
 template <typename T>
 __global__ void __launch_bounds__(256) k_epilogue(Dev<T> D) {
+  // one point per thread (k in the grid); the lowest level's thread also
+  // extrapolates ps, with tvirt of the level above formed again
   COL_IDX
   const int nk = g.nk;
   double tvk = 0, tvkm = 0, pk = 0;
-  for (int k = 1; k <= nk; k++) {
+  {
+    const int k = blockIdx.z + 1;
     long x = g.at(j, i, k);
     double pai = D.pai[x];
     double tv = D.tetav[x] * pai;
@@ -1562,11 +1570,10 @@ __global__ void __launch_bounds__(256) k_epilogue(Dev<T> D) {
     D.p[x] = p_t;
     D.rho[x] = (T)((double)p_t / (kRgas * t));
     D.qsat[x] = (T)d_pfwsat(t, (double)p_t);
-    if (k == nk - 1) tvkm = (double)tv_t;
-    if (k == nk) {
-      tvk = (double)tv_t;
-      pk = (double)p_t;
-    }
+    if (k != nk) return;
+    tvkm = (double)(T)(D.tetav[x - g.plane] * D.pai[x - g.plane]);
+    tvk = (double)tv_t;
+    pk = (double)p_t;
   }
   double zk = (double)D.zeta[g.at(j, i, nk)], zkm = (double)D.zeta[g.at(j, i, nk - 1)];
   double zdgz = zk * kEgrav;
@@ -3928,7 +3935,7 @@ struct Core : CoreBase {
       launch(KK_FILTER_BRACKET, fbytes(0, 3), k_addto<double, double>, pt_grid(), blk(), D.tetav,
              (const double *)D.smt, (const double *)D.tf, g);
     }
-    launch(KK_EPILOGUE, fbytes(9, 3), k_epilogue<T>, col_grid(), blk(), D);
+    launch(KK_EPILOGUE, fbytes(9, 3), k_epilogue<T>, pt_grid(), blk(), D);
     HIP_TRY(hipGetLastError());
     if (retrain_now) {
       // retrain on the fresh pairs, normalisation recomputed, then resume
