@@ -129,10 +129,11 @@ def roofline_of(prof: dict, name: str, traffic: dict) -> dict:
 def step_bytes(kz: int, fp64: bool) -> float:
     """Algorithmic bytes of one column-step of the ordinary step: 470 field
     passes of kz levels (prologue 11, 15 acoustic substeps x 25, 3 advection
-    cycles x 24, epilogue 12); 53 of them are Exner-function passes, which stay
-    FP64 in the FP32 core."""
+    cycles x 24, epilogue 12); 108 of them are passes over the Exner function,
+    theta or the temperature (prologue 3, per substep 6, per advection cycle
+    4, epilogue 3), which stay FP64 in the FP32 core."""
     s = 8 if fp64 else 4
-    return (470 * s + 53 * (8 - s)) * kz
+    return (470 * s + 108 * (8 - s)) * kz
 
 
 def dominant(prof: dict) -> tuple[str, float, float, int]:

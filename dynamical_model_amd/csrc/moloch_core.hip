@@ -3459,7 +3459,9 @@ struct Core : CoreBase {
     }
     for (long r0 = 0; r0 < L.rtot; r0 += L.chunk) {
       const int n = (int)(L.rtot - r0 < L.chunk ? L.rtot - r0 : L.chunk);
-      const double flops = 2.0 * n * (L.nin * nh + nh * nh + nh * L.nout);
+      // algorithmic flops: the 9 x 6 kz real inputs, not the zero padding to nin
+      const double nreal = 9.0 * LAC_IN_FIELDS * g.nk;
+      const double flops = 2.0 * n * (nreal * nh + nh * nh + nh * L.nout);
       if (L.cfg.engine == 1) {
         timed(KK_LAC_MLP, flops, [&]() { fused_forward(r0, n); });
         continue;

@@ -41,7 +41,7 @@ def test_step_bytes():
     FP64: 225.6 kB and 125.5 kB per column-step at kz = 60."""
     b = _bench()
     assert b.step_bytes(60, True) == 470 * 8 * 60 == 225600
-    assert b.step_bytes(60, False) == (470 * 4 + 53 * 4) * 60 == 125520
+    assert b.step_bytes(60, False) == (470 * 4 + 108 * 4) * 60 == 138720
 
 
 def test_committed_traffic_table():
