@@ -7,6 +7,10 @@
 //   7r4w column 7 fields read, 4 written per point, one column per lane
 //               walking k in (64, 4) blocks (the column kernels' shape)
 //   7r4w point  the same with k in the grid (one point per thread)
+//   stagger S   7r4w column and point on the eleven arrays carved out of one
+//               allocation, array q starting q (F + S) bytes in (F the
+//               array's size rounded up to 2 MiB): the rate's dependence on
+//               the arrays' relative placement
 // Every access is within the allocations by construction.
 #include <hip/hip_runtime.h>
 
@@ -110,6 +114,23 @@ int main() {
   timeit("7r4w column (64,4)", Bc, [&] { k_col<<<dim3(NJ / 64, NI / 4), dim3(64, 4)>>>(f); });
   timeit("7r4w point", Bc, [&] { k_pt<<<dim3(NJ / 64, NI / 4, NK), dim3(64, 4)>>>(f); });
   for (int q = 0; q < 11; q++) CK(hipFree(buf[q]));
+  // one slab, array q at q (F + S) bytes
+  const size_t F = (NEL * sizeof(float) + (2u << 20) - 1) / (2u << 20) * (2u << 20);
+  const size_t smax = 262144;
+  char *slab;
+  CK(hipMalloc(&slab, 11 * (F + smax)));
+  CK(hipMemset(slab, 0, 11 * (F + smax)));
+  for (size_t S : {(size_t)0, (size_t)256, (size_t)512, (size_t)1024, (size_t)2048, (size_t)4096, (size_t)8192,
+                   (size_t)16384, (size_t)65536, smax}) {
+    for (int q = 0; q < 7; q++) f.r[q] = (const float *)(slab + q * (F + S));
+    for (int q = 0; q < 4; q++) f.w[q] = (float *)(slab + (7 + q) * (F + S));
+    char nm[64];
+    std::snprintf(nm, sizeof nm, "stagger %zu column", S);
+    timeit(nm, Bc, [&] { k_col<<<dim3(NJ / 64, NI / 4), dim3(64, 4)>>>(f); });
+    std::snprintf(nm, sizeof nm, "stagger %zu point", S);
+    timeit(nm, Bc, [&] { k_pt<<<dim3(NJ / 64, NI / 4, NK), dim3(64, 4)>>>(f); });
+  }
+  CK(hipFree(slab));
   CK(hipFree(out));
   std::printf("done\n");
   return 0;
