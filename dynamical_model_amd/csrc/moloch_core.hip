@@ -1074,15 +1074,24 @@ __device__ __forceinline__ Q waf_vflux(Q s, Q dtrdz, Q qm2, Q qm1, Q q0, Q q1) {
 }
 
 // ============================================= multi-field wafone, 2 kernels
-// The advected set (:911-920) goes through wafone in groups of up to MAXF
-// fields of one storage type; in the FP32 core the FP64 Exner function rides
-// along with the last group in the horizontal kernel (FieldD).  k_wafv runs both vertical half passes for
-// every field of the group, one column per lane, sharing the loads of s,
-// fmz, fmzf and the zrfmu/zrfmd divisions between the fields.  k_wafh then
-// does the meridional and zonal sweeps of every field of the group for one
-// (tile, level) per block, reading u, v, fmz once into LDS.  Same arithmetic
-// as the single-field kernels.
-constexpr int MAXF = 5;
+// The advected set (:911-920) goes through wafone in groups of fields of one
+// storage type (at most waf_gmax<Q>() per group); in the FP32 core the FP64
+// θ and Exner function ride along with the last group in the horizontal
+// kernel (FieldD).  k_wafv runs both vertical half passes for every field of
+// the group, one column per lane, sharing the loads of s, fmz, fmzf and the
+// zrfmu/zrfmd divisions between the fields.  k_wafh then does the meridional
+// and zonal sweeps of every field of the group for one strip of one level
+// per wave, sharing the loads of u, v, fmz and the map factors.  Same
+// arithmetic as the single-field kernels.
+// Group sizes: all eight FP32 fields in one group (the shared loads once per
+// cycle: 2 ms per 2048 x 2048 x 60 step faster than groups of five and three,
+// at 166 VGPRs for the horizontal kernel with its FP64 riders); FP64 fields
+// in groups of five (eight need 212-254 VGPRs).
+constexpr int MAXF = 8;
+template <typename Q>
+constexpr int waf_gmax() {
+  return sizeof(Q) == sizeof(double) ? 5 : 8;
+}
 
 template <typename Q>
 struct FieldSet {
@@ -2839,7 +2848,7 @@ struct Core : CoreBase {
     AF(s, K1) AF(deltaw, K1) AF(zdh, K) AF(wx, K) AF(qwltot, K) AF(qwitot, K);
     if (!segmented_solve() || p.divfilter) AF(zds, K);
     if (p.divfilter) AF(zdf, K);
-    for (int f = 0; f < MAXF; f++) {
+    for (int f = 0; f < waf_gmax<T>(); f++) {
       if ((rc = alloc_field(&gwz[f], K)) != MLC_OK) return rc;
       if ((rc = alloc_field(&gout[f], K)) != MLC_OK) return rc;
     }
@@ -3226,14 +3235,15 @@ struct Core : CoreBase {
     for (int f = 0; f < NF; f++) std::swap(*fld[f], outs[f]);
     for (int d = 0; d < ND; d++) std::swap(*dfld[d], douts[d]);
   }
-  // The n fields in groups of at most MAXF; the nd FP64 riders, if any, go
-  // with the last group.
+  // The n fields in groups of at most waf_gmax<Q>(); the nd FP64 riders, if
+  // any, go with the last group.
   // The first nfree fields are not charged (see waf_group).
   template <typename Q>
   void waf_groups(Q **fld[], int n, Q **outs, Q **wzs, double **dfld[], int nd, double **douts, double **dwzs,
                   double dta, int nfree = 0) {
-    for (int b = 0; b < n; b += MAXF) {
-      const int m = n - b < MAXF ? n - b : MAXF;
+    constexpr int GM = waf_gmax<Q>();
+    for (int b = 0; b < n; b += GM) {
+      const int m = n - b < GM ? n - b : GM;
       const bool rider = nd > 0 && b + m == n;
       const int nf = nfree - b < 0 ? 0 : (nfree - b > m ? m : nfree - b);
 #define MLC_WAF_GROUP(NF)                                                   \
@@ -3249,7 +3259,16 @@ struct Core : CoreBase {
         case 2: MLC_WAF_GROUP(2) break;
         case 3: MLC_WAF_GROUP(3) break;
         case 4: MLC_WAF_GROUP(4) break;
-        default: MLC_WAF_GROUP(5) break;
+        case 5: MLC_WAF_GROUP(5) break;
+        default:
+          if constexpr (GM == 8) {
+            switch (m) {
+              case 6: MLC_WAF_GROUP(6) break;
+              case 7: MLC_WAF_GROUP(7) break;
+              default: MLC_WAF_GROUP(8) break;
+            }
+          }
+          break;
       }
 #undef MLC_WAF_GROUP
     }
