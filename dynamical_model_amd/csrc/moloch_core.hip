@@ -2173,21 +2173,39 @@ __global__ void __launch_bounds__(FT) k_mlp_tail(TailArgs a) {
 
   // ---- h2 = relu(acc + b2) as bf16 into the tile, once every wave is done
   // with the stage buffers (its reads of them have fed its MFMAs; a bare
-  // barrier, so the weight loads above stay in flight)
+  // barrier, so the weight loads above stay in flight).  Lanes 2q and 2q+1
+  // hold outputs n and n+1 of the same rows; they trade one value per pair
+  // of rows (a DPP swap within the lane pair), so that the even lane stores
+  // both outputs of the even row and the odd lane both of the odd row, each
+  // as one 32-bit word (two lanes writing the halves of one word cost LDS
+  // bank conflicts).
   __asm__ volatile("" ::: "memory");
   __builtin_amdgcn_s_barrier();
   __asm__ volatile("" ::: "memory");
+  const int odd = lane & 1;
 #pragma unroll
   for (int x = 0; x < 2; x++)
 #pragma unroll
     for (int y = 0; y < 4; y++) {
       const int n = wn * 128 + 32 * y + (lane & 31);
+      const int ne = n & ~1;  // the pair's even output
       const float bn = a.b2[n];
 #pragma unroll
-      for (int r = 0; r < 16; r++) {
-        const int m = wm * 64 + 32 * x + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const float v = acc[x][y][r] + bn;
-        sH[m * TH + (((n >> 3) ^ (m & 15)) << 3) + (n & 7)] = (__bf16)(v > 0.f ? v : 0.f);
+      for (int r = 0; r < 16; r += 2) {
+        float v0 = acc[x][y][r] + bn, v1 = acc[x][y][r + 1] + bn;
+        v0 = v0 > 0.f ? v0 : 0.f;
+        v1 = v1 > 0.f ? v1 : 0.f;
+        // even lane sends its row r+1 value, odd lane its row r value
+        const float send = odd ? v0 : v1;
+        const float got = __int_as_float(
+            __builtin_amdgcn_update_dpp(0, __float_as_int(send), 0xB1 /* quad_perm 1,0,3,2 */, 0xF, 0xF, false));
+        const float lo = odd ? got : v0, hi = odd ? v1 : got;
+        const int rr = r + odd;
+        const int m = wm * 64 + 32 * x + 8 * (rr >> 2) + 4 * (lane >> 5) + (rr & 3);
+        const __bf16 blo = (__bf16)lo, bhi = (__bf16)hi;
+        const unsigned word = (unsigned)__builtin_bit_cast(unsigned short, blo) |
+                              ((unsigned)__builtin_bit_cast(unsigned short, bhi) << 16);
+        *reinterpret_cast<unsigned *>(&sH[m * TH + (((ne >> 3) ^ (m & 15)) << 3) + (ne & 7)]) = word;
       }
     }
   __syncthreads();
