@@ -39,6 +39,9 @@ struct mlc_mlp {
   __bf16 *H1 = nullptr, *H2 = nullptr, *dH1 = nullptr, *dH2 = nullptr, *dYb = nullptr;
   float *Y = nullptr, *dYf = nullptr, *dH1f = nullptr, *dH2f = nullptr;
   double *partial = nullptr;
+  // split-k partial sums of the gradient products (ws_cap floats)
+  float *ws = nullptr;
+  long ws_cap = 0;
   // weights packed as MFMA B fragments for the fused inference kernels:
   // fragment (ks, nt) holds 64 lanes x 8 bf16, lane = n%32 + 32*((k%16)/8),
   // element e = k%8, for k-step ks = k/16 and n-tile nt = n/32; the n extent

@@ -9,8 +9,12 @@
 //
 // GEMM: C[M x N] = A[M x K] . B[K x N] with arbitrary element strides for A
 // and B (so transposed operands need no copy), on
-// v_mfma_f32_32x32x16_bf16.  A block computes a 128 x 128 tile with four
-// waves of 64 x 64; K advances 32 at a time through LDS.
+// v_mfma_f32_32x32x16_bf16.  k_gemm_v takes the operands that run
+// contiguously in aligned runs of 8 (every product of training and of the
+// generic forward at the corrector's shapes): 64 x 64 tiles, 16-byte loads
+// eight k tiles ahead, and, for the gradient products, a deterministic
+// split of k when the tiles alone give few blocks.  k_gemm_bf16 (128 x 128
+// tiles, element loads) takes any other strides.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -45,6 +49,8 @@ struct GemmArgs {
   __bf16 *Cb;         // bf16 output (ldc), or null
   long ldc;
   int accumulate;     // Cf += result instead of Cf = result
+  float *ws;          // workspace for split-k partial sums (ws_cap floats), or null
+  long ws_cap;
 };
 
 // Threads stage a 128 x 32 tile of A as LDS [m][k] and a 32 x 128 tile of B
@@ -128,10 +134,207 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs a) {
       }
 }
 
+// The training-size GEMMs (1024-4096 rows) on 64 x 64 tiles with 16-byte
+// operand loads.  Each operand runs contiguously either along k or along its
+// other index (m for A, n for B) in aligned runs of 8 elements, which the
+// caller checks (gemm below): a thread loads one run of 8 of each operand
+// per 32-deep k tile -- along k it lands as one 16-byte LDS store in the
+// [row][k] layout of k_gemm_bf16, along m or n as eight 2-byte stores into
+// eight rows -- and runs are either wholly inside the matrix or wholly
+// outside (zeros).  The loads run VD k tiles ahead of the MFMAs in a ring of
+// registers (the global latency of a few blocks per CU is long next to the
+// two k-steps of a tile), the LDS tiles are double-buffered, one barrier
+// per tile.  Four waves of 32 x 32 outputs; every output sees the MFMAs of
+// its k-steps of 16 in increasing order, the fragments of k_gemm_bf16, so
+// without a k split the results are those of k_gemm_bf16 bit for bit.
+// With part != null the block's k tiles are slice blockIdx.z of a split of
+// the k range: it stores its raw partial sums in part[slice][m][n], and
+// k_gemm_reduce adds the slices in slice order (deterministic).
+constexpr int VBM = 64, VBN = 64, VBK = 32, VLDK = VBK + 8, VD = 8;
+
+template <bool AK, bool BK>
+__global__ void __launch_bounds__(256) k_gemm_v(GemmArgs a, int kps, float *part) {
+  __shared__ __attribute__((aligned(16))) __bf16 sA[2][VBM][VLDK];
+  __shared__ __attribute__((aligned(16))) __bf16 sB[2][VBN][VLDK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int m0 = blockIdx.y * VBM, n0 = blockIdx.x * VBN;
+  const int nkt = (a.K + VBK - 1) / VBK;
+  const int kt0 = blockIdx.z * kps;
+  const int kt1 = kt0 + kps < nkt ? kt0 + kps : nkt;
+  // this thread's run of each tile: along k, row tid/4 and k offset
+  // 8 (tid%4) (64 rows x 4 runs); along m or n, k row tid/8 and offset
+  // 8 (tid%8) (32 k x 8 runs)
+  const int r_row = AK ? tid >> 2 : tid >> 3, r_off = AK ? (tid & 3) * 8 : (tid & 7) * 8;
+  const int q_row = BK ? tid >> 2 : tid >> 3, q_off = BK ? (tid & 3) * 8 : (tid & 7) * 8;
+  const bf16x8 zero8 = {};
+  // Loads are unconditional, from addresses clamped into the matrix; a run
+  // outside it is replaced by zeros only when the tile is stored to LDS,
+  // VD tiles later (a select right after the load would wait for it).  A
+  // tile past the slice reloads the slice's last tile (never read).
+  // Branch-free loads let the compiler wait for each ring slot alone.
+  auto clampt = [&](int kt) { return kt < kt1 ? kt : kt1 - 1; };
+  auto okA = [&](int kt) {
+    const int k0 = clampt(kt) * VBK;
+    return AK ? (m0 + r_row < a.M && k0 + r_off < a.K) : (m0 + r_off < a.M && k0 + r_row < a.K);
+  };
+  auto okB = [&](int kt) {
+    const int k0 = clampt(kt) * VBK;
+    return BK ? (n0 + q_row < a.N && k0 + q_off < a.K) : (n0 + q_off < a.N && k0 + q_row < a.K);
+  };
+  auto loadA = [&](int kt) -> bf16x8 {
+    const int k0 = clampt(kt) * VBK;
+    if (AK) {
+      const int m = m0 + r_row, k = k0 + r_off;
+      return *reinterpret_cast<const bf16x8 *>(a.A + (long)(m < a.M ? m : a.M - 1) * a.sam + (k < a.K ? k : a.K - 8));
+    }
+    const int k = k0 + r_row, m = m0 + r_off;
+    return *reinterpret_cast<const bf16x8 *>(a.A + (m < a.M ? m : a.M - 8) + (long)(k < a.K ? k : a.K - 1) * a.sak);
+  };
+  auto loadB = [&](int kt) -> bf16x8 {
+    const int k0 = clampt(kt) * VBK;
+    if (BK) {
+      const int n = n0 + q_row, k = k0 + q_off;
+      return *reinterpret_cast<const bf16x8 *>(a.B + (long)(n < a.N ? n : a.N - 1) * a.sbn + (k < a.K ? k : a.K - 8));
+    }
+    const int k = k0 + q_row, n = n0 + q_off;
+    return *reinterpret_cast<const bf16x8 *>(a.B + (n < a.N ? n : a.N - 8) + (long)(k < a.K ? k : a.K - 1) * a.sbk);
+  };
+  // tile kt's runs (raw loads va, vb) into LDS buffer buf
+  auto store = [&](bf16x8 va, bf16x8 vb, int buf, int kt) {
+    va = okA(kt) ? va : zero8;
+    vb = okB(kt) ? vb : zero8;
+    if (AK) {
+      *reinterpret_cast<bf16x8 *>(&sA[buf][r_row][r_off]) = va;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; e++) sA[buf][r_off + e][r_row] = va[e];
+    }
+    if (BK) {
+      *reinterpret_cast<bf16x8 *>(&sB[buf][q_row][q_off]) = vb;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; e++) sB[buf][q_off + e][q_row] = vb[e];
+    }
+  };
+
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; r++) acc[r] = 0.f;
+  // ring: tile kt in slot (kt - kt0) % VD
+  bf16x8 ra[VD], rb[VD];
+#pragma unroll
+  for (int u = 0; u < VD; u++) {
+    ra[u] = loadA(kt0 + u);
+    rb[u] = loadB(kt0 + u);
+  }
+  store(ra[0], rb[0], 0, kt0);
+  ra[0] = loadA(kt0 + VD);
+  rb[0] = loadB(kt0 + VD);
+  __syncthreads();
+  // tile kt (ring slot u): its two k-steps from LDS buffer (kt - kt0) & 1;
+  // then tile kt+1 into the other buffer (its last reads, tile kt-1, ended
+  // before the previous barrier; after the last tile a copy of it that
+  // nobody reads) and the slot refilled with tile kt+1+VD
+  auto step = [&](int kt, int u) {
+    const int buf = (kt - kt0) & 1;
+#pragma unroll
+    for (int ks = 0; ks < VBK; ks += 16) {
+      const int kq = ks + 8 * (lane >> 5);
+      const bf16x8 av = *reinterpret_cast<const bf16x8 *>(&sA[buf][wm + (lane & 31)][kq]);
+      const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(&sB[buf][wn + (lane & 31)][kq]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+    }
+    store(ra[(u + 1) % VD], rb[(u + 1) % VD], buf ^ 1, kt + 1);
+    ra[(u + 1) % VD] = loadA(kt + 1 + VD);
+    rb[(u + 1) % VD] = loadB(kt + 1 + VD);
+    __syncthreads();
+  };
+  // whole groups of VD tiles in a straight-line body (the ring slots stay
+  // in their registers across the loop), then the fewer than VD left
+  const int nfull = (kt1 - kt0) / VD * VD;
+  int kt = kt0;
+  for (; kt < kt0 + nfull; kt += VD) {
+#pragma unroll
+    for (int u = 0; u < VD; u++) step(kt + u, u);
+  }
+#pragma unroll
+  for (int u = 0; u < VD - 1; u++)
+    if (kt + u < kt1) step(kt + u, u);  // uniform over the block
+
+  // D[m][n]: n = lane % 32, m = 8*(r/4) + 4*(lane/32) + r%4
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    const int m = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    const int n = n0 + wn + (lane & 31);
+    if (m >= a.M || n >= a.N) continue;
+    float v = acc[r];
+    if (part) {
+      part[((long)blockIdx.z * a.M + m) * a.N + n] = v;
+      continue;
+    }
+    if (a.bias) v += a.bias[n];
+    if (a.relu) v = v > 0.f ? v : 0.f;
+    const long o = (long)m * a.ldc + n;
+    if (a.Cf) a.Cf[o] = a.accumulate ? a.Cf[o] + v : v;
+    if (a.Cb) a.Cb[o] = to_bf16(v);
+  }
+}
+
+// Cf = sum of the S slices of part[S][M][N], added in slice order.
+__global__ void __launch_bounds__(256) k_gemm_reduce(const float *part, int S, int M, int N, float *Cf, long ldc,
+                                                     int accumulate) {
+  const long t = blockIdx.x * 256L + threadIdx.x, mn = (long)M * N;
+  if (t >= mn) return;
+  float s = part[t];
+  for (int q = 1; q < S; q++) s += part[q * mn + t];
+  const long o = (t / N) * ldc + t % N;
+  Cf[o] = accumulate ? Cf[o] + s : s;
+}
+
+static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+template <bool AK, bool BK>
+static void launch_gemm_v(hipStream_t st, const GemmArgs &a, dim3 grid, int kps, float *part) {
+  hipLaunchKernelGGL((k_gemm_v<AK, BK>), grid, dim3(256), 0, st, a, kps, part);
+}
+
+// C = A . B.  Operands that run contiguously along k, or along m (A) / n (B)
+// in aligned runs of 8, go to k_gemm_v; others to k_gemm_bf16.  Only a
+// plain fp32 product (no bias, ReLU or bf16 copy: the gradients) with a
+// workspace is split along k, into enough slices to give about 256 blocks.
 static int gemm(hipStream_t st, const GemmArgs &a) {
   if (a.M <= 0 || a.N <= 0 || a.K <= 0) return MLC_OK;
-  dim3 grid((a.N + GBN - 1) / GBN, (a.M + GBM - 1) / GBM);
-  hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, a);
+  const bool ak = a.sak == 1 && a.K % 8 == 0 && a.sam % 8 == 0;
+  const bool am = !ak && a.sam == 1 && a.M % 8 == 0 && a.sak % 8 == 0;
+  const bool bk = a.sbk == 1 && a.K % 8 == 0 && a.sbn % 8 == 0;
+  const bool bn = !bk && a.sbn == 1 && a.N % 8 == 0 && a.sbk % 8 == 0;
+  if ((ak || am) && (bk || bn) && aligned16(a.A) && aligned16(a.B)) {
+    const int gm = (a.M + VBM - 1) / VBM, gn = (a.N + VBN - 1) / VBN, nkt = (a.K + VBK - 1) / VBK;
+    int S = 1;
+    if (a.ws && a.Cf && !a.Cb && !a.bias && !a.relu && (long)gm * gn < 256) {
+      S = (int)((256 + (long)gm * gn - 1) / ((long)gm * gn));
+      S = S < 16 ? S : 16;
+      S = S < nkt / 4 ? S : (nkt / 4 > 1 ? nkt / 4 : 1);
+      while (S > 1 && (long)S * a.M * a.N > a.ws_cap) S--;
+    }
+    const int kps = (nkt + S - 1) / S;
+    S = (nkt + kps - 1) / kps;  // every slice non-empty
+    float *part = S > 1 ? a.ws : nullptr;
+    const dim3 grid((unsigned)gn, (unsigned)gm, (unsigned)S);
+    if (ak && bk) launch_gemm_v<true, true>(st, a, grid, kps, part);
+    else if (ak) launch_gemm_v<true, false>(st, a, grid, kps, part);
+    else if (bk) launch_gemm_v<false, true>(st, a, grid, kps, part);
+    else launch_gemm_v<false, false>(st, a, grid, kps, part);
+    if (part) {
+      const long mn = (long)a.M * a.N;
+      hipLaunchKernelGGL(k_gemm_reduce, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, (const float *)part, S,
+                         a.M, a.N, a.Cf, a.ldc, a.accumulate);
+    }
+  } else {
+    dim3 grid((a.N + GBN - 1) / GBN, (a.M + GBM - 1) / GBM);
+    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, a);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return mlc_fail(MLC_EHIP, "gemm launch: %s", hipGetErrorString(e));
   return MLC_OK;
@@ -301,36 +504,48 @@ int mlp_forward_dev(mlc_mlp *p, const __bf16 *X, long ldx, int rows, float *Yout
 int mlp_backward_dev(mlc_mlp *p, const __bf16 *X, long ldx, const float *T, int rows, double *loss) {
   int rc = mlp_forward_dev(p, X, ldx, rows, nullptr, 0);
   if (rc) return rc;
+  // the gradient products may split k: partial sums in a workspace
+  constexpr long kWsFloats = 2L << 20;
+  if (!p->ws) {
+    if ((rc = mlp_alloc(p, (void **)&p->ws, kWsFloats * sizeof(float)))) return rc;
+    p->ws_cap = kWsFloats;
+  }
+  auto G = [&]() {
+    GemmArgs g{};
+    g.ws = p->ws;
+    g.ws_cap = p->ws_cap;
+    return g;
+  };
   const long ny = (long)rows * p->nout, nh = (long)rows * p->nhid;
   const int nblk = (int)((ny + 255) / 256);
   hipLaunchKernelGGL(k_mse, dim3(nblk), dim3(256), 0, p->stream, p->Y, T, p->dYb, p->dYf, ny,
                      1.0f / (float)ny, p->partial);
   // layer 3: gW3 = H2^T dY, gb3 = colsum dY, dH2 = dY W3^T
-  GemmArgs a{};
+  GemmArgs a = G();
   a.A = p->H2, a.sam = 1, a.sak = p->nhid, a.B = p->dYb, a.sbk = p->nout, a.sbn = 1;
   a.M = p->nhid, a.N = p->nout, a.K = rows, a.Cf = p->gW[2], a.ldc = p->nout;
   if ((rc = gemm(p->stream, a))) return rc;
   hipLaunchKernelGGL(k_colsum, dim3(p->nout), dim3(256), 0, p->stream, p->dYf, p->gb[2], rows, p->nout);
-  a = GemmArgs{};
+  a = G();
   a.A = p->dYb, a.sam = p->nout, a.sak = 1, a.B = p->Wb[2], a.sbk = 1, a.sbn = p->nout;
   a.M = rows, a.N = p->nhid, a.K = p->nout, a.Cf = p->dH2f, a.ldc = p->nhid;
   if ((rc = gemm(p->stream, a))) return rc;
   hipLaunchKernelGGL(k_relu_bwd, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, p->stream, p->dH2, p->H2,
                      p->dH2f, nh);
   // layer 2
-  a = GemmArgs{};
+  a = G();
   a.A = p->H1, a.sam = 1, a.sak = p->nhid, a.B = p->dH2, a.sbk = p->nhid, a.sbn = 1;
   a.M = p->nhid, a.N = p->nhid, a.K = rows, a.Cf = p->gW[1], a.ldc = p->nhid;
   if ((rc = gemm(p->stream, a))) return rc;
   hipLaunchKernelGGL(k_colsum, dim3(p->nhid), dim3(256), 0, p->stream, p->dH2f, p->gb[1], rows, p->nhid);
-  a = GemmArgs{};
+  a = G();
   a.A = p->dH2, a.sam = p->nhid, a.sak = 1, a.B = p->Wb[1], a.sbk = 1, a.sbn = p->nhid;
   a.M = rows, a.N = p->nhid, a.K = p->nhid, a.Cf = p->dH1f, a.ldc = p->nhid;
   if ((rc = gemm(p->stream, a))) return rc;
   hipLaunchKernelGGL(k_relu_bwd, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, p->stream, p->dH1, p->H1,
                      p->dH1f, nh);
   // layer 1
-  a = GemmArgs{};
+  a = G();
   a.A = X, a.sam = 1, a.sak = ldx, a.B = p->dH1, a.sbk = p->nhid, a.sbn = 1;
   a.M = p->nin, a.N = p->nhid, a.K = rows, a.Cf = p->gW[0], a.ldc = p->nhid;
   if ((rc = gemm(p->stream, a))) return rc;
