@@ -2086,10 +2086,18 @@ struct TailArgs {
   int nrt;                      // row tiles of TM
 };
 
+// Chunk swizzle of row m of k_mlp_tail's h2 tile: bits 0 and 2 of m go to
+// bits 2 and 3, bits 1 and 3 to bits 0 and 1.
+__device__ __forceinline__ int h2_swz(int m) {
+  return ((m & 1) << 2) | (((m >> 2) & 1) << 3) | ((m >> 1) & 1) | (((m >> 3) & 1) << 1);
+}
+
 __global__ void __launch_bounds__(FT) k_mlp_tail(TailArgs a) {
   // h2 tile [TM][TH] bf16; the 16-byte chunk c of row m is stored at
-  // c ^ (m & 15), so the A-fragment reads of 16 consecutive rows hit
-  // distinct banks
+  // c ^ h2_swz(m), a permutation of the bits of m & 15: the layer-3
+  // A-fragment reads of 16 consecutive rows hit 16 distinct chunks, and the
+  // four rows m, m+1, m+4, m+5 (m even) that one h2 store instruction
+  // writes land in four different groups of four chunks, i.e. distinct banks
   __shared__ __attribute__((aligned(16))) __bf16 sH[TM * TH];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nb = gridDim.x, b = blockIdx.x;
@@ -2205,7 +2213,7 @@ __global__ void __launch_bounds__(FT) k_mlp_tail(TailArgs a) {
         const __bf16 blo = (__bf16)lo, bhi = (__bf16)hi;
         const unsigned word = (unsigned)__builtin_bit_cast(unsigned short, blo) |
                               ((unsigned)__builtin_bit_cast(unsigned short, bhi) << 16);
-        *reinterpret_cast<unsigned *>(&sH[m * TH + (((ne >> 3) ^ (m & 15)) << 3) + (ne & 7)]) = word;
+        *reinterpret_cast<unsigned *>(&sH[m * TH + (((ne >> 3) ^ h2_swz(m)) << 3) + (ne & 7)]) = word;
       }
     }
   __syncthreads();
@@ -2226,7 +2234,7 @@ __global__ void __launch_bounds__(FT) k_mlp_tail(TailArgs a) {
 #pragma unroll
     for (int x = 0; x < 2; x++) {
       const int m = wm * 64 + 32 * x + (lane & 31);
-      const int c = (2 * t + (lane >> 5)) ^ (m & 15);
+      const int c = (2 * t + (lane >> 5)) ^ h2_swz(m);
       av[x] = *reinterpret_cast<const lac_bf16x8 *>(&sH[m * TH + c * 8]);
     }
 #pragma unroll
