@@ -93,6 +93,28 @@ def test_mlp_forward_and_gradients_match_reference(nin, nhid, nout, rows):
     net.close()
 
 
+def test_gradients_are_deterministic():
+    """The gradient products at the corrector's shape split k into slices
+    (partial sums added in slice order by a second kernel), so the same batch
+    gives the same bits every time; the forward pass, which never splits,
+    gives the same bits too.  kz = 60 shape: 3312 -> 512 -> 512 -> 240."""
+    rng = np.random.default_rng(11)
+    net = MLP(3312, 512, 240, seed=9)
+    x = rng.normal(size=(1024, 3312)).astype(np.float32)
+    t = rng.normal(size=(1024, 240)).astype(np.float32)
+    runs = []
+    for _ in range(3):
+        loss = net.grad(x, t)
+        runs.append((loss, [net.gradients(l) for l in range(3)], net.forward(x)))
+    for loss, grads, y in runs[1:]:
+        assert loss == runs[0][0]
+        assert np.array_equal(y, runs[0][2])
+        for (gw, gb), (rw, rb) in zip(grads, runs[0][1]):
+            assert np.array_equal(gw, rw) and np.array_equal(gb, rb)
+    assert all(np.isfinite(g).all() and np.abs(g).max() > 0 for g, _ in runs[0][1])
+    net.close()
+
+
 def test_adam_steps_match_numpy():
     """Two Adam steps (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected)
     against float64 numpy from the engine's own gradients: the moments, the
