@@ -65,3 +65,45 @@ def test_cpu_baseline_sample():
     assert rate > 0 and dt > 0
     rate_t, _ = b.cpu_baseline(16, 16, 10, 1, threaded=True)
     assert rate_t > 0
+
+
+def test_failed_learned_variant_leaves_the_line(monkeypatch, capsys):
+    """A library error in the learned variant is recorded in the line, and
+    the FP64 and FP32 measurements (value from the FP32 core) still stand; an
+    error in a core variant still ends the run.  The variants are stand-ins
+    (no GPU): fixed timings, kernel profiles and fields."""
+    import json
+    import types
+
+    import numpy as np
+
+    import dynamical_model_amd as dm
+
+    b = _bench()
+    fields = {f: np.full((3, 8, 8), 1.0 + i) for i, f in enumerate(b.PROGNOSTIC)}
+
+    def fake(dm_, hs, case, precision, steps, warmup, lead, learned=None):
+        if learned is not None:
+            raise dm.MolochError("mlc_lac_create", 4, "out of device memory (stand-in)")
+        secs = 0.2 if precision == dm.MLC_FP32 else 0.4
+        prof = {"sound_b": (secs * 400.0, 15 * steps, 1.0e10 * 15 * steps), "waf_h": (secs * 100.0, 3 * steps, 1e9)}
+        diag = types.SimpleNamespace(ps_max=1.0, ps_min=0.5, u_max=1.0, u_min=0.0, v_max=1.0, v_min=0.0,
+                                     cfl_violation=0)
+        return secs, prof, {f: a.copy() for f, a in fields.items()}, diag, {}
+
+    monkeypatch.setattr(b, "run_variant", fake)
+    monkeypatch.setattr(sys, "argv", ["bench.py", "--steps", "2", "--jx", "16", "--iy", "16", "--kz", "4",
+                                      "--cpu-sample", "none", "--cpu-sample-threaded", "none"])
+    assert b.main() == 0
+    line = json.loads(capsys.readouterr().out.strip().splitlines()[-1])
+    assert "learned" in line["failed_variants"] and "learned" not in line["variants"]
+    assert line["config"]["variant"] == "fp32"
+    assert line["value"] == pytest.approx(16 * 16 * 2 / 0.2)
+    assert line["roofline"]["kernel"] == "sound_b" and line["mlp_roofline"] is None
+
+    def fake_core_error(*args, **kwargs):
+        raise dm.MolochError("mlc_create", 4, "stand-in")
+
+    monkeypatch.setattr(b, "run_variant", fake_core_error)
+    with pytest.raises(dm.MolochError):
+        b.main()
